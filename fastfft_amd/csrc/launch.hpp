@@ -1,0 +1,41 @@
+// Host-side launch interface of the line kernels (implemented in launch_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "plan.hpp"
+
+namespace fastfft_amd {
+
+// Element types of the operands of one stage launch.
+enum class RealT { f32 = 0, f16 = 1 };
+enum class CpxT { c64 = 0, c32h = 1 };
+
+struct StageIO {
+    const void* src = nullptr;
+    void*       dst = nullptr;
+    const void* img = nullptr; // xcorr only
+    bool        src_real = false, dst_real = false;
+    RealT       src_rt = RealT::f32, dst_rt = RealT::f32;
+    CpxT        src_ct = CpxT::c64, dst_ct = CpxT::c64, img_ct = CpxT::c64;
+    float       scale = 1.f; // pre (fwd), post (inv), intra (xcorr)
+};
+
+hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
+hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
+hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
+
+// Length of the twiddle table (the largest supported line transform).
+constexpr int kTwiddleTableN = 8192;
+
+// Fills tw[k] = exp(-2 pi i k / kTwiddleN) (FP64 on the GPU, rounded to FP32).
+hipError_t launch_twiddle_init(float2* tw, hipStream_t stream);
+
+// Argmax of a real [ny][pitch] image: writes {value, x, y} to result (device memory).
+hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* result_value, int* result_xy,
+                         hipStream_t stream);
+
+// LDS bytes per workgroup of a line stage with sub-transform size P.
+size_t stage_lds_bytes(int P);
+
+} // namespace fastfft_amd

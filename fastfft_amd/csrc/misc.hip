@@ -1,0 +1,110 @@
+// Twiddle-table generation (the only FP64 step) and the argmax epilogue.
+#include "launch_common.hpp"
+
+namespace fastfft_amd {
+
+__global__ void k_twiddle_init(float2* __restrict__ tw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( k < kTwiddleN ) {
+        double s, c;
+        // exp(-2 pi i k / NT): angle in units of pi is 2k/NT, exact in binary
+        sincospi(2.0 * (double)k / (double)kTwiddleN, &s, &c);
+        tw[k] = make_float2((float)c, (float)(-s));
+    }
+}
+
+hipError_t launch_twiddle_init(float2* tw, hipStream_t stream) {
+    k_twiddle_init<<<dim3((kTwiddleN + 255) / 256), dim3(256), 0, stream>>>(tw);
+    return hipGetLastError( );
+}
+
+// ---------------------------------------------------------------- argmax
+// (value, linear index) pairs ordered by value, then by smaller index, so the
+// first maximum in row-major order wins, as a sequential scan would.
+struct Best {
+    float v;
+    int   i;
+};
+
+__device__ __forceinline__ Best better(Best a, Best b) {
+    if ( b.v > a.v || (b.v == a.v && b.i < a.i) )
+        return b;
+    return a;
+}
+
+constexpr int kArgmaxThreads = 256;
+constexpr int kArgmaxBlocks  = 512;
+
+__device__ __forceinline__ Best block_reduce(Best b) {
+    __shared__ float sv[kArgmaxThreads];
+    __shared__ int   si[kArgmaxThreads];
+    sv[threadIdx.x] = b.v;
+    si[threadIdx.x] = b.i;
+    __syncthreads( );
+    for ( int s = kArgmaxThreads / 2; s > 0; s >>= 1 ) {
+        if ( threadIdx.x < s ) {
+            Best o   = {sv[threadIdx.x + s], si[threadIdx.x + s]};
+            Best m   = {sv[threadIdx.x], si[threadIdx.x]};
+            Best r   = better(m, o);
+            sv[threadIdx.x] = r.v;
+            si[threadIdx.x] = r.i;
+        }
+        __syncthreads( );
+    }
+    Best r = {sv[0], si[0]};
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kArgmaxThreads)
+        k_argmax_partial(const T* __restrict__ img, int nx, int ny, int pitch, float* __restrict__ pv, int* __restrict__ pi) {
+    const long n  = (long)nx * ny;
+    Best       b  = {-INFINITY, 0x7fffffff};
+    for ( long i = (long)blockIdx.x * kArgmaxThreads + threadIdx.x; i < n; i += (long)gridDim.x * kArgmaxThreads ) {
+        const int y = (int)(i / nx), x = (int)(i % nx);
+        Best      c = {load_real(img, (long)y * pitch + x), (int)i};
+        b           = better(b, c);
+    }
+    b = block_reduce(b);
+    if ( threadIdx.x == 0 ) {
+        pv[blockIdx.x] = b.v;
+        pi[blockIdx.x] = b.i;
+    }
+}
+
+__global__ void __launch_bounds__(kArgmaxThreads)
+        k_argmax_final(const float* __restrict__ pv, const int* __restrict__ pi, int nparts, int nx, float* __restrict__ rv,
+                       int* __restrict__ rxy) {
+    Best b = {-INFINITY, 0x7fffffff};
+    for ( int i = threadIdx.x; i < nparts; i += kArgmaxThreads ) {
+        Best c = {pv[i], pi[i]};
+        b      = better(b, c);
+    }
+    b = block_reduce(b);
+    if ( threadIdx.x == 0 ) {
+        rv[0]  = b.v;
+        rxy[0] = b.i % nx;
+        rxy[1] = b.i / nx;
+    }
+}
+
+// result_value: 1 + kArgmaxBlocks floats of device scratch, result_xy: 2 + kArgmaxBlocks ints.
+hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* result_value, int* result_xy,
+                         hipStream_t stream) {
+    const long n      = (long)nx * ny;
+    int        blocks = (int)std::min<long>(kArgmaxBlocks, (n + kArgmaxThreads * 4 - 1) / (kArgmaxThreads * 4));
+    blocks            = blocks < 1 ? 1 : blocks;
+    float* pv         = result_value + 1;
+    int*   pi         = result_xy + 2;
+    if ( rt == RealT::f32 )
+        k_argmax_partial<float><<<blocks, kArgmaxThreads, 0, stream>>>(static_cast<const float*>(img), nx, ny, pitch, pv, pi);
+    else
+        k_argmax_partial<__half><<<blocks, kArgmaxThreads, 0, stream>>>(static_cast<const __half*>(img), nx, ny, pitch, pv, pi);
+    hipError_t e = hipGetLastError( );
+    if ( e != hipSuccess )
+        return e;
+    k_argmax_final<<<1, kArgmaxThreads, 0, stream>>>(pv, pi, blocks, nx, result_value, result_xy);
+    return hipGetLastError( );
+}
+
+} // namespace fastfft_amd

@@ -1,0 +1,166 @@
+/*
+ * fastfft.h -- C ABI of the MI355X-native FastFFT size-changing transform path.
+ *
+ * Plain C: opaque plan handle, sizes and device pointers passed as void*,
+ * no HIP or torch types in any signature (streams travel as void* holding a
+ * hipStream_t).  Every entry point returns a fastfft_status; the message of
+ * the last failure on the calling thread is in fastfft_last_error_string().
+ *
+ * Each entry point replaces one member of the reference's
+ * FastFFT::FourierTransformer<ComputeBaseType, InputType, OtherImageType, Rank>
+ * (include/FastFFT.h:84-732, src/fastfft/FastFFT.cu); the citation is given
+ * with each declaration.  The C++ drop-in header include/fastfft/FastFFT.h and
+ * the Python package fastfft_amd (ctypes) are both thin layers over this ABI.
+ *
+ * Data layout contract (byte-identical to the reference):
+ *   real buffers   FFTW padded: rows of 2*(x/2+1) elements (InputType)
+ *   2-D spectrum   C[kx][ky]: x/2+1 rows of y complex values, ky contiguous
+ *   3-D spectrum   C[kz][kx][ky]
+ *   transforms     unnormalised (Inv(Fwd(x)) = Nx*Ny*Nz * x)
+ *   CONJ_MUL       image_to_search * conj(FFT(input))
+ */
+#ifndef FASTFFT_AMD_FASTFFT_H
+#define FASTFFT_AMD_FASTFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fastfft_status {
+    FASTFFT_SUCCESS           = 0,
+    FASTFFT_INVALID_ARGUMENT  = 1, /* null handle/pointer, bad enum */
+    FASTFFT_INVALID_DIMENSION = 2, /* the reference's runtime asserts in ValidateDimensions */
+    FASTFFT_UNSUPPORTED       = 3, /* valid request with no kernel path (e.g. 3-D fwd decrease) */
+    FASTFFT_NOT_PLANNED       = 4, /* a transform called before both plans were set */
+    FASTFFT_HIP_ERROR         = 5, /* a HIP runtime call failed */
+    FASTFFT_OUT_OF_MEMORY     = 6,
+    FASTFFT_LDS_EXCEEDED      = 7  /* a stage would need more than the 160 KiB of LDS per CU */
+} fastfft_status;
+
+/* InputType of the reference template (real position-space data). */
+typedef enum fastfft_input_type { FASTFFT_INPUT_F32 = 0, FASTFFT_INPUT_F16 = 1 } fastfft_input_type;
+
+/* OtherImageType: storage of image_to_search in FwdImageInvFFT. */
+typedef enum fastfft_image_type { FASTFFT_IMAGE_C64 = 0 /* float2 */, FASTFFT_IMAGE_C32H = 1 /* __half2 */ } fastfft_image_type;
+
+/* Element-wise operators fused into the transforms; the reference's functors
+ * KernelFunction::my_functor<T, N, IKF_t> (include/detail/functors.h:15-80). */
+typedef enum fastfft_op_kind {
+    FASTFFT_OP_NOOP                = 0,
+    FASTFFT_OP_SCALE               = 1, /* pre/post op: x * scale */
+    FASTFFT_OP_CONJ_MUL            = 2, /* intra op: image * conj(transform) */
+    FASTFFT_OP_CONJ_MUL_THEN_SCALE = 3  /* intra op: image * conj(transform) * scale */
+} fastfft_op_kind;
+
+typedef struct fastfft_op {
+    int   kind;  /* fastfft_op_kind */
+    float scale; /* used by SCALE and CONJ_MUL_THEN_SCALE */
+} fastfft_op;
+
+/* Which dimensions fastfft_get_dims reports. */
+typedef enum fastfft_dims_which {
+    FASTFFT_FWD_INPUT  = 0,
+    FASTFFT_FWD_OUTPUT = 1,
+    FASTFFT_INV_INPUT  = 2,
+    FASTFFT_INV_OUTPUT = 3
+} fastfft_dims_which;
+
+typedef struct fastfft_plan_s* fastfft_plan;
+
+/* FourierTransformer() constructor (FastFFT.cu:58-75).  compute type is always
+ * FP32 (the reference's static_assert at :63).  No GPU work happens here. */
+fastfft_status fastfft_create(fastfft_plan* plan, int input_type, int image_type, int rank);
+
+/* ~FourierTransformer() / Deallocate (FastFFT.cu:77-113): frees the workspace. */
+fastfft_status fastfft_destroy(fastfft_plan plan);
+
+/* SetForwardFFTPlan (FastFFT.cu:132-161, FastFFT.h:102-104). */
+fastfft_status fastfft_set_fwd_plan(fastfft_plan plan, size_t in_x, size_t in_y, size_t in_z, size_t out_x, size_t out_y,
+                                    size_t out_z, int is_padded_input);
+
+/* SetInverseFFTPlan (FastFFT.cu:180-200, FastFFT.h:106-108). */
+fastfft_status fastfft_set_inv_plan(fastfft_plan plan, size_t in_x, size_t in_y, size_t in_z, size_t out_x, size_t out_y,
+                                    size_t out_z, int is_padded_output);
+
+/* Extension (no reference counterpart): origin of the window kept by a trimmed
+ * inverse, as wrapped lags.  Default (0,0,0) is the reference's top-left trim
+ * (FastFFT.cu:1342, :1675); (-out_x/2, -out_y/2, -out_z/2) centres it on lag 0. */
+fastfft_status fastfft_set_output_window(fastfft_plan plan, int origin_x, int origin_y, int origin_z);
+
+/* Stream for all work of this plan (the reference always uses
+ * cudaStreamPerThread, docs/_docs/references/usage.md:13).  stream is a
+ * hipStream_t; NULL selects the per-thread default stream. */
+fastfft_status fastfft_set_stream(fastfft_plan plan, void* stream);
+
+/* FwdFFT(input, output = nullptr, pre_op, intra_op) (FastFFT.cu:263-277, :315-459).
+ * out == NULL transforms in place. */
+fastfft_status fastfft_fwd(fastfft_plan plan, void* in, void* out, fastfft_op pre_op, fastfft_op intra_op);
+
+/* InvFFT(input, output = nullptr, intra_op, post_op) (FastFFT.cu:279-293, :461-607). */
+fastfft_status fastfft_inv(fastfft_plan plan, void* in, void* out, fastfft_op intra_op, fastfft_op post_op);
+
+/* FwdImageInvFFT(input, image_to_search, output = nullptr, pre, intra, post)
+ * (FastFFT.cu:295-313, :609-798).  intra must be CONJ_MUL or CONJ_MUL_THEN_SCALE
+ * (the reference's EnableIf<HasIntraOpFunctor<IntraOpType>>, FastFFT.h:668). */
+fastfft_status fastfft_fwd_image_inv(fastfft_plan plan, void* in, const void* image_to_search, void* out, fastfft_op pre_op,
+                                     fastfft_op intra_op, fastfft_op post_op);
+
+/* Wait() (FastFFT.h:124-126): synchronise the plan's stream. */
+fastfft_status fastfft_wait(fastfft_plan plan);
+
+/* CopyDeviceToHostAndSynchronize(host, n = 0) (FastFFT.cu:969-1009): copies n
+ * InputType elements (0 = the size of the last completed stage) from the buffer
+ * holding the last result. */
+fastfft_status fastfft_copy_device_to_host_and_synchronize(fastfft_plan plan, void* host, int n_elements);
+
+/* ReturnInputMemorySize / ReturnFwdOutputMemorySize / ReturnInvOutputMemorySize
+ * (FastFFT.h:194-204): element counts (InputType elements) of FFTW padded buffers. */
+fastfft_status fastfft_get_memory_sizes(fastfft_plan plan, int* input, int* fwd_output, int* inv_output);
+
+/* ReturnFwdInputDimensions ... ReturnInvOutputDimensions (FastFFT.h:206-220):
+ * dims[0..3] = x, y, z, w (w = complex elements along x after R2C). */
+fastfft_status fastfft_get_dims(fastfft_plan plan, int which, int dims[4]);
+
+/* GetDeviceBufferPointer (FastFFT.h:128-130): the plan's internal workspace
+ * (NULL until the first transform allocates it). */
+fastfft_status fastfft_get_buffer_pointer(fastfft_plan plan, void** ptr);
+
+/* Bytes of device workspace the plan allocates at its first transform. */
+fastfft_status fastfft_workspace_bytes(fastfft_plan plan, size_t* bytes);
+
+/* Extension: the argmax epilogue of the correlation path.  img is a real
+ * [ny][pitch] device array of InputType (as written by FwdImageInvFFT);
+ * peak[0..1] receive (x, y) of the first maximum in row-major order, value
+ * its value (float).  peak and value are host pointers; the call synchronises. */
+fastfft_status fastfft_argmax(fastfft_plan plan, const void* img, int nx, int ny, int pitch, int peak[2], float* value);
+
+/* Extension: the argmax epilogue without a host round trip.  Writes (x, y) to
+ * device int[2] peak_xy and the value to device float value, asynchronously on
+ * the plan's stream. */
+fastfft_status fastfft_argmax_device(fastfft_plan plan, const void* img, int nx, int ny, int pitch, int* peak_xy, float* value);
+
+/* Runs only stages [first, last] of an operation ("fwd", "inv" or "xcorr"),
+ * the run-time counterpart of the reference's compile-time FFT_DEBUG_STAGE
+ * partial transforms (src/tests/helper_functions.cuh:125-198).  Intermediate
+ * stages read and write the plan's workspace; stage layouts are listed by
+ * fastfft_describe.  Used for per-stage layout tests and per-kernel timing. */
+fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, int last, void* in, const void* image_to_search,
+                                  void* out, fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op);
+
+/* Human-readable description of the stage sequence an operation would run
+ * ("fwd", "inv" or "xcorr"), written without touching the GPU. */
+fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, size_t buf_len);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fastfft_last_error_string(void);
+
+/* Library version string. */
+const char* fastfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTFFT_AMD_FASTFFT_H */

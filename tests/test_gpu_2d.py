@@ -1,0 +1,302 @@
+"""GPU parity of the 2-D FourierTransformer path against the FP64 oracle.
+
+Runs through the C ABI (ctypes binding) on cuda:0.  Tolerances: relative L2
+error <= 2e-6 for FP32 against the FP64 oracle (the stated FP32 bar), exact
+equality for the reference's closed-form known-answer tests (the reference
+itself compares with == and != 0.0f, src/tests/helper_functions.cuh:91-109),
+and exact argmax agreement for correlations.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import gpu_helpers as H
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 2e-6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def rand_img(x, y, seed):
+    return np.random.default_rng(seed).standard_normal((y, x)).astype(np.float32)
+
+
+def run_fwd(ft, host_padded, nfloats):
+    buf = H.dev_zeros(nfloats)
+    buf[: host_padded.size] = H.to_dev(host_padded.reshape(-1))
+    ft.FwdFFT(buf)
+    return buf
+
+
+# ------------------------------------------------------------------ round trip
+def test_round_trip_64_into_512():
+    H.log("round trip 64x64 -> 512x512 fwd, 512 inv")
+    img = rand_img(64, 64, seed=1)
+    ft = H.make_plan((64, 64, 1), (512, 512, 1), (512, 512, 1), (512, 512, 1))
+    n = max(ft.ReturnInputMemorySize(), ft.ReturnFwdOutputMemorySize(), ft.ReturnInvOutputMemorySize())
+    buf = run_fwd(ft, H.padded_real(img, 64), n)
+    spec = H.complex_from_f32(H.to_host(buf), (257, 512))
+    want = O.fwd_2d(H.padded_real(img, 64), 64, 512, 512)
+    assert O.rel_l2(spec, want) < TOL
+    ft.InvFFT(buf)
+    back = H.real_window(H.to_host(buf), 512, 512)
+    want_back = O.inv_2d(H.complex_from_f32(H.complex_to_f32(spec), (257, 512)), 512, 512, 512, 512)
+    assert O.rel_l2(back, want_back) < TOL
+    padded = np.zeros((512, 512))
+    padded[:64, :64] = img
+    assert O.rel_l2(back / (512 * 512), padded) < TOL
+
+
+# ------------------------------------------------------------------ golden fixtures through the GPU
+def test_golden_forward(golden):
+    for key in ("bf", "np"):
+        in_x, in_y, n_x, n_y = (int(v) for v in golden[f"{key}_fwd_dims"])
+        H.log(f"golden fwd {key} {in_x}x{in_y} -> {n_x}x{n_y}")
+        ft = H.make_plan((in_x, in_y, 1), (n_x, n_y, 1), (n_x, n_y, 1), (n_x, n_y, 1))
+        n = max(ft.ReturnInputMemorySize(), ft.ReturnFwdOutputMemorySize())
+        buf = run_fwd(ft, golden[f"{key}_fwd_in"], n)
+        got = H.complex_from_f32(H.to_host(buf), (n_x // 2 + 1, n_y))
+        assert O.rel_l2(got, golden[f"{key}_fwd_out"]) < TOL
+
+
+def test_golden_inverse(golden):
+    for key in ("bf", "np"):
+        n_x, n_y, out_x, out_y, ox, oy = (int(v) for v in golden[f"{key}_inv_dims"])
+        H.log(f"golden inv {key} {n_x}x{n_y} -> {out_x}x{out_y} at ({ox},{oy})")
+        ft = H.make_plan((n_x, n_y, 1), (n_x, n_y, 1), (n_x, n_y, 1), (out_x, out_y, 1), origin=(ox, oy, 0))
+        spec = golden[f"{key}_inv_in"]
+        inp = H.to_dev(H.complex_to_f32(spec))
+        out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+        ft.InvFFT(inp, out)
+        got = H.real_window(H.to_host(out), out_x, out_y)
+        want = golden[f"{key}_inv_out"]
+        # the fixture was made from the FP64 spectrum; compare with the oracle on the FP32-rounded input too
+        want32 = O.inv_2d(spec.astype(np.complex64), n_x, n_y, out_x, out_y, ox, oy)
+        assert O.rel_l2(got, want32) < TOL
+        assert O.rel_l2(got, want) < 1e-5
+
+
+def test_golden_xcorr(golden):
+    tx, ty, n_x, n_y, out_x, out_y, ox, oy = (int(v) for v in golden["xc_dims"])
+    H.log("golden xcorr")
+    ft = H.make_plan((tx, ty, 1), (n_x, n_y, 1), (n_x, n_y, 1), (out_x, out_y, 1), origin=(ox, oy, 0))
+    t = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+    tin = golden["xc_tmpl"].reshape(-1)
+    t[: tin.size] = H.to_dev(tin)
+    img = H.to_dev(H.complex_to_f32(golden["xc_image_spec"]))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFT(t, img, out)
+    got = H.real_window(H.to_host(out), out_x, out_y)
+    want32 = O.xcorr_2d(golden["xc_tmpl"], tx, golden["xc_image_spec"].astype(np.complex64), n_x, n_y, out_x, out_y, ox, oy)
+    assert O.rel_l2(got, want32) < TOL
+    assert O.rel_l2(got, golden["xc_out"]) < 1e-5
+
+
+# ------------------------------------------------------------------ the reference's known-answer tests
+SIZES = [32, 64, 128, 256, 512, 1024, 2048, 4096]  # src/tests/tests.h:11
+
+
+@pytest.mark.parametrize("inplace", [True, False])
+def test_kat_unit_impulse_increase(inplace):
+    # src/tests/unit_impulse_test.cu:4-218 (increase, in place and out of place)
+    for i, small in enumerate(SIZES[:-1]):
+        for big in SIZES[i + 1:]:
+            H.log(f"unit impulse {small} -> {big} inplace={inplace}")
+            ft = H.make_plan((small, small, 1), (big, big, 1), (big, big, 1), (big, big, 1))
+            n = max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize(), ft.ReturnFwdOutputMemorySize())
+            img = np.zeros((small, small), np.float32)
+            img[0, 0] = 1.0
+            a = H.dev_zeros(n)
+            a[: small * H.F.padded_pitch(small)] = H.to_dev(H.padded_real(img, small).reshape(-1))
+            if inplace:
+                ft.FwdFFT(a)
+                spec_buf = a
+            else:
+                b = H.dev_zeros(n)
+                ft.FwdFFT(a, b)
+                a.zero_()
+                spec_buf = b
+            spec = H.complex_from_f32(H.to_host(spec_buf), (big // 2 + 1, big))
+            assert np.sum(np.abs(spec)) == (big // 2 + 1) * big
+            if inplace:
+                ft.InvFFT(a)
+                out = a
+            else:
+                ft.InvFFT(spec_buf, a)
+                out = a
+            back = H.real_window(H.to_host(out), big, big)
+            assert back.sum() == big * big
+            assert back[0, 0] == big * big
+
+
+def test_kat_unit_impulse_decrease():
+    # unit_impulse_test.cu:29-38 with do_increase_size = false: Fourier cropping
+    for i, small in enumerate(SIZES[:-1]):
+        for big in SIZES[i + 1:]:
+            H.log(f"unit impulse decrease {big} -> {small}")
+            ft = H.make_plan((big, big, 1), (small, small, 1), (small, small, 1), (small, small, 1))
+            n = max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize(), ft.ReturnFwdOutputMemorySize())
+            img = np.zeros((big, big), np.float32)
+            img[0, 0] = 1.0
+            a = H.dev_zeros(n)
+            a[:] = H.to_dev(H.padded_real(img, big).reshape(-1))
+            ft.FwdFFT(a)
+            spec = H.complex_from_f32(H.to_host(a), (small // 2 + 1, small))
+            assert np.sum(np.abs(spec)) == (small // 2 + 1) * small
+            ft.InvFFT(a)
+            back = H.real_window(H.to_host(a), small, small)
+            assert back.sum() == small * small
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+def test_kat_constant_image(n):
+    # src/tests/constant_image_test.cu:87-101, :186-192
+    H.log(f"constant image {n}")
+    ft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1))
+    a = H.dev_zeros(ft.ReturnInputMemorySize())
+    a[:] = H.to_dev(H.padded_real(np.ones((n, n), np.float32), n).reshape(-1))
+    ft.FwdFFT(a)
+    spec = H.complex_from_f32(H.to_host(a), (n // 2 + 1, n))
+    assert spec[0, 0] == n * n
+    spec[0, 0] = 0
+    assert np.all(spec == 0)
+    ft.InvFFT(a)
+    back = H.real_window(H.to_host(a), n, n)
+    assert back.sum() == float(n * n) * (n * n)
+
+
+CONV_PAIRS = [(16, 64), (64, 64), (64, 512), (128, 1024), (256, 4096), (512, 4096), (384, 4096)]
+
+
+@pytest.mark.parametrize("small,big", CONV_PAIRS)
+def test_kat_impulse_convolution(small, big):
+    # src/tests/padded_convolution_FastFFT_vs_cuFFT.cu:179-265: 2 (*) 3 -> N*6 at the origin, exact zeros elsewhere
+    H.log(f"impulse convolution {small} -> {big}")
+    ft = H.make_plan((small, small, 1), (big, big, 1), (big, big, 1), (big, big, 1))
+    tgt = H.make_plan((big, big, 1), (big, big, 1), (big, big, 1), (big, big, 1))
+    t = np.zeros((big, big), np.float32)
+    t[0, 0] = 3.0
+    tb = H.dev_zeros(tgt.ReturnFwdOutputMemorySize())
+    tb[:] = H.to_dev(H.padded_real(t, big).reshape(-1))
+    tgt.FwdFFT(tb)
+    x = np.zeros((small, small), np.float32)
+    x[0, 0] = 2.0
+    n = max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize())
+    a = H.dev_zeros(n)
+    a[: small * H.F.padded_pitch(small)] = H.to_dev(H.padded_real(x, small).reshape(-1))
+    ft.FwdImageInvFFT(a, tb, a, intra_op=H.F.ConjMul())
+    out = H.real_window(H.to_host(a), big, big)
+    assert out[0, 0] == big * big * 6.0
+    out[0, 0] = 0
+    assert np.all(out == 0.0)
+    # SCALE pre-op and CONJ_MUL_THEN_SCALE give exactly 1.0 (preop_functor_scale_test.cu:229-364)
+    s = 1.0 / (big * big * 6.0)
+    for pre, intra in ((H.F.Scale(s), H.F.ConjMul()), (H.F.Noop(), H.F.ConjMulThenScale(s))):
+        a.zero_()
+        a[: small * H.F.padded_pitch(small)] = H.to_dev(H.padded_real(x, small).reshape(-1))
+        ft.FwdImageInvFFT(a, tb, a, pre_op=pre, intra_op=intra)
+        out = H.real_window(H.to_host(a), big, big)
+        assert out[0, 0] == 1.0
+
+
+# ------------------------------------------------------------------ random-data parity sweep
+FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (256, 128)), ((512, 512), (512, 512)),
+             ((4096, 4096), (4096, 4096)), ((8192, 16), (8192, 16)), ((16, 8192), (32, 8192)), ((1000, 900), (1024, 1024)),
+             ((256, 256), (64, 32))]
+
+
+@pytest.mark.parametrize("fin,fout", FWD_CASES)
+def test_fwd_random(fin, fout):
+    H.log(f"fwd random {fin} -> {fout}")
+    img = rand_img(fin[0], fin[1], seed=fin[0] + 7 * fin[1])
+    ft = H.make_plan((*fin, 1), (*fout, 1), (*fout, 1), (*fout, 1))
+    n = max(ft.ReturnInputMemorySize(), ft.ReturnFwdOutputMemorySize())
+    buf = run_fwd(ft, H.padded_real(img, fin[0]), n)
+    got = H.complex_from_f32(H.to_host(buf), (fout[0] // 2 + 1, fout[1]))
+    want = O.fwd_2d(H.padded_real(img, fin[0]), fin[0], fout[0], fout[1])
+    assert O.rel_l2(got, want) < TOL
+
+
+INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256, -256)), ((4096, 4096), (512, 512), (0, 0)),
+             ((256, 128), (100, 60), (-50, -30)), ((64, 2048), (64, 300), (0, 7)), ((8192, 32), (8192, 32), (0, 0)),
+             ((1024, 1024), (1000, 1000), (3, -5))]
+
+
+@pytest.mark.parametrize("nin,nout,origin", INV_CASES)
+def test_inv_random(nin, nout, origin):
+    H.log(f"inv random {nin} -> {nout} at {origin}")
+    rng = np.random.default_rng(nin[0] + nout[1])
+    w = nin[0] // 2 + 1
+    spec = (rng.standard_normal((w, nin[1])) + 1j * rng.standard_normal((w, nin[1]))).astype(np.complex64)
+    ft = H.make_plan((*nin, 1), (*nin, 1), (*nin, 1), (*nout, 1), origin=(*origin, 0))
+    inp = H.to_dev(H.complex_to_f32(spec))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.InvFFT(inp, out)
+    got = H.real_window(H.to_host(out), nout[0], nout[1])
+    want = O.inv_2d(spec, nin[0], nin[1], nout[0], nout[1], *origin)
+    assert O.rel_l2(got, want) < TOL
+
+
+XC_CASES = [((64, 64), (512, 512), (512, 512), (0, 0)), ((128, 128), (1024, 1024), (256, 256), (-128, -128)),
+            ((96, 80), (256, 256), (64, 128), (-32, -64)), ((512, 512), (512, 512), (512, 512), (0, 0)),
+            ((512, 512), (512, 512), (128, 128), (0, 0)), ((100, 50), (2048, 512), (300, 100), (-150, -50))]
+
+
+@pytest.mark.parametrize("tin,n,nout,origin", XC_CASES)
+def test_xcorr_random(tin, n, nout, origin):
+    H.log(f"xcorr random {tin} in {n} -> {nout} at {origin}")
+    rng = np.random.default_rng(tin[0] * 3 + n[1])
+    t = rng.standard_normal((tin[1], tin[0])).astype(np.float32)
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((w, n[1])) + 1j * rng.standard_normal((w, n[1]))).astype(np.complex64)
+    ft = H.make_plan((*tin, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+    tp = H.padded_real(t, tin[0]).reshape(-1)
+    a[: tp.size] = H.to_dev(tp)
+    img = H.to_dev(H.complex_to_f32(spec))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFT(a, img, out)
+    got = H.real_window(H.to_host(out), nout[0], nout[1])
+    want = O.xcorr_2d(H.padded_real(t, tin[0]), tin[0], spec, n[0], n[1], nout[0], nout[1], *origin)
+    assert O.rel_l2(got, want) < TOL
+
+
+# ------------------------------------------------------------------ the headline correlation, full size
+def test_xcorr_4096_384_trim_512_centred():
+    from fastfft_amd import workloads as W
+
+    H.log("config-2 correlation 4096^2 x 384^2 -> 512^2 centred")
+    mic, tmpl, lags = W.xcorr_config()
+    # micrograph spectrum on the GPU (no size change), checked against the oracle
+    mft = H.make_plan((4096, 4096, 1), (4096, 4096, 1), (4096, 4096, 1), (4096, 4096, 1))
+    mbuf = H.dev_zeros(mft.ReturnFwdOutputMemorySize())
+    mbuf[:] = H.to_dev(W.to_padded(mic).reshape(-1))
+    mft.FwdFFT(mbuf)
+    mspec = H.complex_from_f32(H.to_host(mbuf), (2049, 4096))
+    want_spec = O.fwd_2d(W.to_padded(mic), 4096, 4096, 4096)
+    assert O.rel_l2(mspec, want_spec) < TOL
+    ft = H.make_plan((384, 384, 1), (4096, 4096, 1), (4096, 4096, 1), (512, 512, 1), origin=(-256, -256, 0))
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+    tp = W.to_padded(tmpl).reshape(-1)
+    a[: tp.size] = H.to_dev(tp)
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFT(a, mbuf, out)
+    got = H.real_window(H.to_host(out), 512, 512)
+    want = O.xcorr_2d(W.to_padded(tmpl), 384, mspec, 4096, 4096, 512, 512, -256, -256)
+    err = O.rel_l2(got, want)
+    print(f"config-2 rel_l2={err:.3e} max_abs={O.max_abs(got, want):.3e} max={np.abs(want).max():.3e}")
+    assert err < TOL
+    gx, gy, _ = O.argmax_2d(got)
+    wx, wy, _ = O.argmax_2d(want)
+    assert (gx, gy) == (wx, wy)
+    assert (gx - 256, gy - 256) == lags[0]
+    # the library's own argmax epilogue agrees
+    px, py, pv = ft.ArgMax(out, 512, 512, H.F.padded_pitch(512))
+    assert (px, py) == (gx, gy)
