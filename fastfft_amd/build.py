@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 SOURCES = ["planner.cpp", "capi.cpp", "launch_fwd.hip", "launch_inv.hip", "launch_xcorr.hip", "misc.hip"]
-HEADERS = ["fft_core.hpp", "line_kernels.hpp", "launch.hpp", "launch_common.hpp", "plan.hpp"]
+HEADERS = ["fft_core.hpp", "line_kernels.hpp", "launch.hpp", "launch_common.hpp", "plan.hpp", "wave_fft.hpp",
+           "xcorr_wave.hpp"]
 
 
 def _needs(obj: str, src: str) -> bool:

@@ -1,6 +1,7 @@
 // Shared helpers of the launch_*.hip translation units.
 #pragma once
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "launch.hpp"
@@ -56,5 +57,12 @@ hipError_t prepare_launch(K kernel, int nlines, dim3& grid, dim3& block, size_t&
 }
 
 inline int twiddle_stride(int N) { return kTwiddleN / N; }
+
+// True unless the environment variable `name` is set to a non-empty value other than "0"
+// (used to switch a specialised kernel off and fall back to the generic one).
+inline bool getenv_flag_off(const char* name) {
+    const char* v = std::getenv(name);
+    return ! (v && v[0] && ! (v[0] == '0' && v[1] == 0));
+}
 
 } // namespace fastfft_amd

@@ -1,5 +1,11 @@
 // Fused correlation pass of FwdImageInvFFT.
+//
+// Contiguous lines (the 2-D second axis) with P in {256, 512, 1024}, a window
+// origin that is a multiple of 64 placed at register rotation 0 or E/2, and
+// N = P*Q <= 8192 run the one-wave-per-line kernel of xcorr_wave.hpp; every
+// other shape runs the generic line kernel.
 #include "launch_common.hpp"
+#include "xcorr_wave.hpp"
 
 namespace fastfft_amd {
 
@@ -17,7 +23,91 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
     return hipGetLastError( );
 }
 
+// q batch held in registers: 32 complex values per lane
+template <int P, int Q>
+constexpr int wave_qb( ) {
+    constexpr int E  = P / 64;
+    constexpr int qb = 16 / E;
+    return Q < qb ? Q : qb;
+}
+
+template <int P, int Q, int QB, int RS, typename TImg>
+static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    k_xcorr_wave<P, Q, QB, RS, TImg><<<dim3((unsigned)s.nlines), dim3(64), 0, st>>>(
+            static_cast<const float2*>(io.src), s.im.s1, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
+            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw);
+    return hipGetLastError( );
+}
+
+// Batch size of q sub-transforms held in registers at once.  The default comes
+// from the measured table in the design notes; FASTFFT_XCORR_QB overrides it
+// (1, 2 or 4) for tuning runs.
+template <int P, int Q, int RS, typename TImg>
+static hipError_t go_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    constexpr int QBD = wave_qb<P, Q>( );
+    const char*   env = std::getenv("FASTFFT_XCORR_QB");
+    const int     qb  = env ? std::atoi(env) : QBD;
+    if constexpr ( Q % 4 == 0 ) {
+        if ( qb == 4 )
+            return go_xcorr_wave_qb<P, Q, 4, RS, TImg>(s, io, tw, st);
+    }
+    if constexpr ( Q % 2 == 0 ) {
+        if ( qb == 2 )
+            return go_xcorr_wave_qb<P, Q, 2, RS, TImg>(s, io, tw, st);
+    }
+    if ( qb == 1 )
+        return go_xcorr_wave_qb<P, Q, 1, RS, TImg>(s, io, tw, st);
+    return go_xcorr_wave_qb<P, Q, QBD, RS, TImg>(s, io, tw, st);
+}
+
+template <int P, typename TImg>
+static hipError_t try_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    constexpr int E = P / 64;
+    if ( s.o % 64 != 0 )
+        return hipErrorNotSupported;
+    const int rs = (((s.o / 64) % E) + E) % E;
+    auto      by_q = [&](auto qc) -> hipError_t {
+        constexpr int Q = decltype(qc)::value;
+        if constexpr ( P * Q > kTwiddleN ) {
+            return hipErrorNotSupported;
+        }
+        else {
+            if ( rs == 0 )
+                return go_xcorr_wave<P, Q, 0, TImg>(s, io, tw, st);
+            if ( rs == E / 2 )
+                return go_xcorr_wave<P, Q, E / 2, TImg>(s, io, tw, st);
+            return hipErrorNotSupported;
+        }
+    };
+    switch ( s.Q ) {
+        case 2: return by_q(std::integral_constant<int, 2>{ });
+        case 4: return by_q(std::integral_constant<int, 4>{ });
+        case 8: return by_q(std::integral_constant<int, 8>{ });
+        case 16: return by_q(std::integral_constant<int, 16>{ });
+        default: return hipErrorNotSupported;
+    }
+}
+
+static bool contiguous_lines(const HostLineMap& m) { return m.es == 1 && m.n2 == 1 && m.s2 == 0; }
+
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const bool wave_ok = contiguous_lines(s.im) && contiguous_lines(s.gm) && contiguous_lines(s.om) &&
+                         (io.img_ct == CpxT::c32h || s.gm.s1 % 2 == 0) && getenv_flag_off("FASTFFT_NO_WAVE_XCORR");
+    if ( wave_ok ) {
+        hipError_t e = hipErrorNotSupported;
+        auto       pick = [&](auto pc) -> hipError_t {
+            constexpr int P = decltype(pc)::value;
+            return io.img_ct == CpxT::c64 ? try_xcorr_wave<P, float2>(s, io, tw, st) : try_xcorr_wave<P, __half2>(s, io, tw, st);
+        };
+        if ( s.P == 256 )
+            e = pick(std::integral_constant<int, 256>{ });
+        else if ( s.P == 512 )
+            e = pick(std::integral_constant<int, 512>{ });
+        else if ( s.P == 1024 )
+            e = pick(std::integral_constant<int, 1024>{ });
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     return dispatch_P(s.P, [&](auto pc) -> hipError_t {
         constexpr int P = decltype(pc)::value;
         if ( io.img_ct == CpxT::c64 )

@@ -1,0 +1,150 @@
+// Fused correlation pass, one wavefront per line (the hot kernel of
+// FwdImageInvFFT's second axis; replaces the reference's
+// block_fft_kernel_C2C_FWD_INCREASE_OP_INV_NONE / _FWD_NONE_INV_DECREASE_ConjMul,
+// src/fastfft/FastFFT.cu:20-56, :1295-1344, for the padded-and-trimmed case).
+//
+// For one line (a kx column of the transposed spectrum) of full length N = P*Q:
+//   X[Q k' + q] = FFT_P( a[n] W_N^{n q} )[k'],          n < L  (input pruned)
+//   Z[k]        = img[k] * conj(X[k]) * s                         (CONJ_MUL[_THEN_SCALE])
+//   out[j]      = sum_q W_N^{-(o+j) q} IFFT_P( Z[Q s + q] )[(o + j) mod P],  j < M
+// The wave holds all Q sub-transforms of the line in registers (lane t owns
+// k' = t + 64 i for every q), so the image is read as QB consecutive complex
+// values per (lane, i), the q-sum stays in registers and no workgroup barrier
+// is ever needed.  The window origin o must be a multiple of 64; its register
+// rotation RS = (o/64) mod E is a template parameter.
+#pragma once
+
+#include <utility>
+
+#include "wave_fft.hpp"
+
+namespace fastfft_amd {
+
+template <int... Is, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, Is...>, F&& f) {
+    (f(std::integral_constant<int, Is>{ }), ...);
+}
+
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N-1>)
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(std::make_integer_sequence<int, N>{ }, f);
+}
+
+template <int QB, typename TImg>
+struct ImgLoad;
+
+template <int QB>
+struct ImgLoad<QB, float2> {
+    __device__ __forceinline__ static void run(const float2* __restrict__ p, float2 (&g)[QB]) {
+        if constexpr ( QB % 2 == 0 ) {
+            const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+            for ( int b = 0; b < QB / 2; b++ ) {
+                float4 f     = p4[b];
+                g[2 * b]     = make_float2(f.x, f.y);
+                g[2 * b + 1] = make_float2(f.z, f.w);
+            }
+        }
+        else {
+#pragma unroll
+            for ( int b = 0; b < QB; b++ )
+                g[b] = p[b];
+        }
+    }
+};
+
+template <int QB>
+struct ImgLoad<QB, __half2> {
+    __device__ __forceinline__ static void run(const __half2* __restrict__ p, float2 (&g)[QB]) {
+#pragma unroll
+        for ( int b = 0; b < QB; b++ )
+            g[b] = __half22float2(p[b]);
+    }
+};
+
+template <int P, int Q, int QB, int RS, typename TImg>
+__global__ void __launch_bounds__(64)
+        k_xcorr_wave(const float2* __restrict__ a, long lda, int L, const TImg* __restrict__ img, long ldg,
+                     float2* __restrict__ out, long ldo, int M, int o, float intra_scale, const float2* __restrict__ tw) {
+    constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
+    static_assert(Q % QB == 0, "q batches must tile Q");
+    static_assert(RS >= 0 && RS < E, "register rotation out of range");
+    __shared__ float2 lds[lds_slots(P)];
+    const int  t   = threadIdx.x;
+    const long col = blockIdx.x;
+
+    WaveFFT<P> fft;
+    fft.init(t, tw);
+
+    float2 x[E], acc[E];
+    const float2* ac = a + col * lda;
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int n = t + T * i;
+        x[i]        = n < L ? ac[n] : make_float2(0.f, 0.f);
+        acc[i]      = make_float2(0.f, 0.f);
+    }
+    const TImg* gc    = img + col * ldg;
+    const int   omask = ((o % N) + N) % N; // o mod N
+
+#pragma unroll 1
+    for ( int q0 = 0; q0 < Q; q0 += QB ) {
+        float2 v[QB][E];
+        float2 lw[QB]; // W_N^{t q}
+        // input twiddle W_N^{(t + 64 i) q} = W_N^{t q} * W_N^{64 i q}; the second factor is
+        // wave-uniform and comes through the scalar cache
+#pragma unroll
+        for ( int b = 0; b < QB; b++ ) {
+            const int q = q0 + b;
+            lw[b]       = tw[((t * q) & (N - 1)) * TWS];
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                const float2 c = tw[((T * i * q) & (N - 1)) * TWS];
+                v[b][i]        = cmul(cmul(x[i], c), lw[b]);
+            }
+        }
+#pragma unroll
+        for ( int b = 0; b < QB; b++ )
+            fft.template run<-1>(v[b], lds, t);
+        // conj-mul with the image: Z = g * conj(X)
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int k1 = t + T * i;
+            float2    g[QB];
+            ImgLoad<QB, TImg>::run(gc + (long)Q * k1 + q0, g);
+#pragma unroll
+            for ( int b = 0; b < QB; b++ )
+                v[b][i] = cmulc(g[b], v[b][i]);
+        }
+#pragma unroll
+        for ( int b = 0; b < QB; b++ )
+            fft.template run<+1>(v[b], lds, t);
+        // accumulate: output register i' takes the IFFT value of register (i' + RS) mod E
+        // times W_N^{-(o + t + 64 i') q} = conj(W_N^{o q} W_N^{t q}) * conj(W_N^{64 i' q})
+#pragma unroll
+        for ( int b = 0; b < QB; b++ ) {
+            const int q  = q0 + b;
+            float2    cq = tw[((omask * q) & (N - 1)) * TWS]; // W_N^{o q}
+            float2    s  = cmul(lw[b], cq);                   // W_N^{(o + t) q}
+            s            = make_float2(s.x * intra_scale, -s.y * intra_scale);
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                float2 c = tw[((T * i * q) & (N - 1)) * TWS];
+                c.y      = -c.y;
+                float2 u = cmul(v[b][(i + RS) % E], c);
+                acc[i].x = fmaf(u.x, s.x, fmaf(-u.y, s.y, acc[i].x));
+                acc[i].y = fmaf(u.x, s.y, fmaf(u.y, s.x, acc[i].y));
+            }
+        }
+    }
+    float2* oc = out + col * ldo;
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int j = t + T * i;
+        if ( j < M )
+            oc[j] = acc[i];
+    }
+}
+
+} // namespace fastfft_amd

@@ -268,6 +268,37 @@ def test_xcorr_random(tin, n, nout, origin):
     assert O.rel_l2(got, want) < TOL
 
 
+# the one-wave-per-column kernel (P in {256, 512, 1024}, origin a multiple of 64) against the
+# oracle, with every register batch size and with the generic kernel forced, so both paths stay pinned
+WAVE_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((48, 384), (64, 4096), (32, 512), (0, 0)),
+              ((40, 200), (64, 2048), (24, 256), (-12, -128)), ((32, 300), (32, 8192), (32, 512), (0, -256)),
+              ((24, 1000), (32, 4096), (16, 1024), (0, -512)), ((16, 130), (32, 1024), (16, 256), (0, 128))]
+
+
+@pytest.mark.parametrize("mode", ["default", "qb1", "qb2", "qb4", "generic"])
+@pytest.mark.parametrize("tin,n,nout,origin", WAVE_CASES)
+def test_xcorr_wave_kernel(tin, n, nout, origin, mode, monkeypatch):
+    env = {"default": {}, "qb1": {"FASTFFT_XCORR_QB": "1"}, "qb2": {"FASTFFT_XCORR_QB": "2"},
+           "qb4": {"FASTFFT_XCORR_QB": "4"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    H.log(f"xcorr wave {tin} in {n} -> {nout} at {origin} mode={mode}")
+    rng = np.random.default_rng(tin[1] * 7 + n[1])
+    t = rng.standard_normal((tin[1], tin[0])).astype(np.float32)
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((w, n[1])) + 1j * rng.standard_normal((w, n[1]))).astype(np.complex64)
+    ft = H.make_plan((*tin, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+    tp = H.padded_real(t, tin[0]).reshape(-1)
+    a[: tp.size] = H.to_dev(tp)
+    img = H.to_dev(H.complex_to_f32(spec))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFT(a, img, out, intra_op=H.F.ConjMulThenScale(0.5))
+    got = H.real_window(H.to_host(out), nout[0], nout[1])
+    want = O.xcorr_2d(H.padded_real(t, tin[0]), tin[0], spec, n[0], n[1], nout[0], nout[1], *origin, intra_scale=0.5)
+    assert O.rel_l2(got, want) < TOL
+
+
 # ------------------------------------------------------------------ the headline correlation, full size
 def test_xcorr_4096_384_trim_512_centred():
     from fastfft_amd import workloads as W
