@@ -75,7 +75,33 @@ struct fastfft_plan_s {
     // CopyDeviceToHostAndSynchronize bookkeeping (FastFFT.cu:947-1009)
     void* last_buffer     = nullptr;
     int   stage_completed = 0; // 0: none, 3: forward, 7: inverse / correlation
+    // stage lists per operation (fwd, inv, xcorr), built once per validated plan
+    std::vector<Stage> stages[3];
+    bool               stages_built[3] = {false, false, false};
+    float2*            tw              = nullptr; // twiddle table of ws_device
 };
+
+static int op_index(const char* op) {
+    if ( ! op )
+        return -1;
+    if ( std::strcmp(op, "fwd") == 0 )
+        return 0;
+    if ( std::strcmp(op, "inv") == 0 )
+        return 1;
+    if ( std::strcmp(op, "xcorr") == 0 )
+        return 2;
+    return -1;
+}
+
+static const char* kOpNames[3] = {"fwd", "inv", "xcorr"};
+
+static void invalidate(fastfft_plan p) {
+    p->st.validated = false;
+    for ( int i = 0; i < 3; i++ ) {
+        p->stages_built[i] = false;
+        p->stages[i].clear( );
+    }
+}
 
 static size_t elem_bytes(const PlanState& s) { return s.input_type == FASTFFT_INPUT_F16 ? 2 : 4; }
 
@@ -95,13 +121,18 @@ static fastfft_status ensure_validated(fastfft_plan p) {
 }
 
 static fastfft_status ensure_device_state(fastfft_plan p, float2** tw) {
-    hipError_t e = twiddles_for_current_device(tw, p->stream);
-    if ( e != hipSuccess )
-        return hip_fail(e, "twiddle table");
-    int dev = 0;
-    e       = hipGetDevice(&dev);
+    int        dev = 0;
+    hipError_t e   = hipGetDevice(&dev);
     if ( e != hipSuccess )
         return hip_fail(e, "hipGetDevice");
+    if ( p->ws && p->ws_device == dev && p->tw ) {
+        *tw = p->tw;
+        return ok( );
+    }
+    e = twiddles_for_current_device(tw, p->stream);
+    if ( e != hipSuccess )
+        return hip_fail(e, "twiddle table");
+    p->tw = *tw;
     if ( ! p->ws || p->ws_device != dev ) {
         if ( p->ws ) {
             (void)hipFree(p->ws);
@@ -136,19 +167,33 @@ static float op_scale(const fastfft_op& op) {
     return (op.kind == FASTFFT_OP_SCALE || op.kind == FASTFFT_OP_CONJ_MUL_THEN_SCALE) ? op.scale : 1.f;
 }
 
-static fastfft_status run_stages(fastfft_plan p, const char* opname, void* in, void* out, const void* image, float pre,
-                                 float intra, float post, int first = 0, int last = -1) {
-    std::vector<Stage> stages;
-    PlanError          pe = build_stages(p->st, opname, stages);
-    if ( pe.code )
-        return fail(pe.code, pe.msg);
+static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stage>** out) {
+    if ( ! p->stages_built[opi] ) {
+        std::vector<Stage> st;
+        PlanError          pe = build_stages(p->st, kOpNames[opi], st);
+        if ( pe.code )
+            return fail(pe.code, pe.msg);
+        for ( const Stage& s : st )
+            if ( stage_lds_bytes(s.P) > kMaxLdsBytes )
+                return fail(FASTFFT_LDS_EXCEEDED, "stage " + s.ref_kernel + " needs more LDS than a gfx950 CU has");
+        p->stages[opi]       = std::move(st);
+        p->stages_built[opi] = true;
+    }
+    *out = &p->stages[opi];
+    return ok( );
+}
+
+static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, const void* image, float pre, float intra,
+                                 float post, int first = 0, int last = -1) {
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(p, opi, &sp) )
+        return rc;
+    const std::vector<Stage>& stages = *sp;
     if ( last < 0 )
         last = (int)stages.size( ) - 1;
     if ( first < 0 || last >= (int)stages.size( ) || first > last )
-        return fail(FASTFFT_INVALID_ARGUMENT, "stage range outside the " + std::to_string(stages.size( )) + " stages of " + opname);
-    for ( const Stage& s : stages )
-        if ( stage_lds_bytes(s.P) > kMaxLdsBytes )
-            return fail(FASTFFT_LDS_EXCEEDED, "stage " + s.ref_kernel + " needs more LDS than a gfx950 CU has");
+        return fail(FASTFFT_INVALID_ARGUMENT,
+                    "stage range outside the " + std::to_string(stages.size( )) + " stages of " + kOpNames[opi]);
     float2*        tw = nullptr;
     fastfft_status rc = ensure_device_state(p, &tw);
     if ( rc )
@@ -252,7 +297,7 @@ static fastfft_status set_dims(fastfft_plan plan, Dims& din, Dims& dout, size_t 
     dout.x = (int)ox;
     dout.y = (int)oy;
     dout.z = (int)oz;
-    plan->st.validated = false;
+    invalidate(plan);
     return ok( );
 }
 
@@ -293,6 +338,7 @@ fastfft_status fastfft_set_output_window(fastfft_plan plan, int origin_x, int or
     plan->st.origin[0] = origin_x;
     plan->st.origin[1] = origin_y;
     plan->st.origin[2] = origin_z;
+    invalidate(plan);
     return ok( );
 }
 
@@ -315,7 +361,7 @@ fastfft_status fastfft_fwd(fastfft_plan plan, void* in, void* out, fastfft_op pr
     if ( auto rc = ensure_validated(plan) )
         return rc;
     void* dst = out ? out : in;
-    if ( auto rc = run_stages(plan, "fwd", in, dst, nullptr, op_scale(pre_op), 1.f, 1.f) )
+    if ( auto rc = run_stages(plan, 0, in, dst, nullptr, op_scale(pre_op), 1.f, 1.f) )
         return rc;
     plan->last_buffer     = dst;
     plan->stage_completed = 3;
@@ -334,7 +380,7 @@ fastfft_status fastfft_inv(fastfft_plan plan, void* in, void* out, fastfft_op in
     if ( auto rc = ensure_validated(plan) )
         return rc;
     void* dst = out ? out : in;
-    if ( auto rc = run_stages(plan, "inv", in, dst, nullptr, 1.f, 1.f, op_scale(post_op)) )
+    if ( auto rc = run_stages(plan, 1, in, dst, nullptr, 1.f, 1.f, op_scale(post_op)) )
         return rc;
     plan->last_buffer     = dst;
     plan->stage_completed = 7;
@@ -356,7 +402,7 @@ fastfft_status fastfft_fwd_image_inv(fastfft_plan plan, void* in, const void* im
     if ( auto rc = ensure_validated(plan) )
         return rc;
     void* dst = out ? out : in;
-    if ( auto rc = run_stages(plan, "xcorr", in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op)) )
+    if ( auto rc = run_stages(plan, 2, in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op)) )
         return rc;
     plan->last_buffer     = dst;
     plan->stage_completed = 7;
@@ -440,29 +486,40 @@ fastfft_status fastfft_workspace_bytes(fastfft_plan plan, size_t* bytes) {
     return ok( );
 }
 
-fastfft_status fastfft_argmax(fastfft_plan plan, const void* img, int nx, int ny, int pitch, int peak[2], float* value) {
-    if ( auto rc = check_plan(plan) )
-        return rc;
-    if ( ! img || ! peak || nx <= 0 || ny <= 0 || pitch < nx )
-        return fail(FASTFFT_INVALID_ARGUMENT, "argmax: null pointer or bad extent");
+// Scratch of the argmax epilogue: partial values and indices, then the result slots.
+static fastfft_status argmax_scratch(fastfft_plan plan, float** pv, int** pi, float** rv, int** rxy) {
     if ( ! plan->argmax_scratch ) {
-        hipError_t e = hipMalloc(&plan->argmax_scratch, sizeof(float) * (1 + 512) + sizeof(int) * (2 + 512));
+        hipError_t e = hipMalloc(&plan->argmax_scratch, sizeof(float) * (kArgmaxPartials + 1) + sizeof(int) * (kArgmaxPartials + 2));
         if ( e != hipSuccess ) {
             plan->argmax_scratch = nullptr;
             return hip_fail(e, "hipMalloc(argmax scratch)");
         }
     }
-    float*     rv = plan->argmax_scratch;
-    int*       rx = reinterpret_cast<int*>(rv + 1 + 512);
-    hipError_t e  = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, rv, rx,
-                                  plan->stream);
+    *pv  = plan->argmax_scratch;
+    *rv  = *pv + kArgmaxPartials;
+    *pi  = reinterpret_cast<int*>(*rv + 1);
+    *rxy = *pi + kArgmaxPartials;
+    return ok( );
+}
+
+fastfft_status fastfft_argmax(fastfft_plan plan, const void* img, int nx, int ny, int pitch, int peak[2], float* value) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! img || ! peak || nx <= 0 || ny <= 0 || pitch < nx )
+        return fail(FASTFFT_INVALID_ARGUMENT, "argmax: null pointer or bad extent");
+    float *pv, *rv;
+    int *  pi, *rxy;
+    if ( auto rc = argmax_scratch(plan, &pv, &pi, &rv, &rxy) )
+        return rc;
+    hipError_t e = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, pv, pi,
+                                 rv, rxy, plan->stream);
     if ( e != hipSuccess )
         return hip_fail(e, "argmax launch");
     float hv = 0.f;
     int   hxy[2];
     e = hipMemcpyAsync(&hv, rv, sizeof(float), hipMemcpyDeviceToHost, plan->stream);
     if ( e == hipSuccess )
-        e = hipMemcpyAsync(hxy, rx, 2 * sizeof(int), hipMemcpyDeviceToHost, plan->stream);
+        e = hipMemcpyAsync(hxy, rxy, 2 * sizeof(int), hipMemcpyDeviceToHost, plan->stream);
     if ( e == hipSuccess )
         e = hipStreamSynchronize(plan->stream);
     if ( e != hipSuccess )
@@ -479,21 +536,12 @@ fastfft_status fastfft_argmax_device(fastfft_plan plan, const void* img, int nx,
         return rc;
     if ( ! img || ! peak_xy || ! value || nx <= 0 || ny <= 0 || pitch < nx )
         return fail(FASTFFT_INVALID_ARGUMENT, "argmax: null pointer or bad extent");
-    if ( ! plan->argmax_scratch ) {
-        hipError_t e = hipMalloc(&plan->argmax_scratch, sizeof(float) * (1 + 512) + sizeof(int) * (2 + 512));
-        if ( e != hipSuccess ) {
-            plan->argmax_scratch = nullptr;
-            return hip_fail(e, "hipMalloc(argmax scratch)");
-        }
-    }
-    float*     rv = plan->argmax_scratch;
-    int*       rx = reinterpret_cast<int*>(rv + 1 + 512);
-    hipError_t e  = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, rv, rx,
-                                  plan->stream);
-    if ( e == hipSuccess )
-        e = hipMemcpyAsync(value, rv, sizeof(float), hipMemcpyDeviceToDevice, plan->stream);
-    if ( e == hipSuccess )
-        e = hipMemcpyAsync(peak_xy, rx, 2 * sizeof(int), hipMemcpyDeviceToDevice, plan->stream);
+    float *pv, *rv;
+    int *  pi, *rxy;
+    if ( auto rc = argmax_scratch(plan, &pv, &pi, &rv, &rxy) )
+        return rc;
+    hipError_t e = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, pv, pi,
+                                 value, peak_xy, plan->stream);
     return e == hipSuccess ? ok( ) : hip_fail(e, "argmax");
 }
 
@@ -501,12 +549,12 @@ fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, 
                                   void* out, fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op) {
     if ( auto rc = check_plan(plan) )
         return rc;
-    if ( ! op || ! in )
-        return fail(FASTFFT_INVALID_ARGUMENT, "null op or input pointer");
-    const std::string o(op);
-    if ( o == "xcorr" && ! image_to_search )
+    const int opi = op_index(op);
+    if ( opi < 0 || ! in )
+        return fail(FASTFFT_INVALID_ARGUMENT, "unknown op (expected fwd, inv or xcorr) or null input pointer");
+    if ( opi == 2 && ! image_to_search )
         return fail(FASTFFT_INVALID_ARGUMENT, "xcorr stages need image_to_search");
-    if ( o == "xcorr" && intra_op.kind != FASTFFT_OP_CONJ_MUL && intra_op.kind != FASTFFT_OP_CONJ_MUL_THEN_SCALE )
+    if ( opi == 2 && intra_op.kind != FASTFFT_OP_CONJ_MUL && intra_op.kind != FASTFFT_OP_CONJ_MUL_THEN_SCALE )
         return fail(FASTFFT_INVALID_ARGUMENT, "xcorr stages need an intra op (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
     if ( auto rc = op_kind_check(pre_op, true, false, "a pre op") )
         return rc;
@@ -515,7 +563,7 @@ fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, 
     if ( auto rc = ensure_validated(plan) )
         return rc;
     void* dst = out ? out : in;
-    return run_stages(plan, op, in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), first, last);
+    return run_stages(plan, opi, in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), first, last);
 }
 
 fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, size_t buf_len) {
@@ -525,11 +573,13 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
         return fail(FASTFFT_INVALID_ARGUMENT, "describe: null argument");
     if ( auto rc = ensure_validated(plan) )
         return rc;
-    std::vector<Stage> stages;
-    PlanError          pe = build_stages(plan->st, op, stages);
-    if ( pe.code )
-        return fail(pe.code, pe.msg);
-    std::string s = describe(plan->st, stages);
+    const int opi = op_index(op);
+    if ( opi < 0 )
+        return fail(FASTFFT_INVALID_ARGUMENT, std::string("unknown operation '") + op + "' (expected fwd, inv or xcorr)");
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(plan, opi, &sp) )
+        return rc;
+    std::string s = describe(plan->st, *sp);
     std::snprintf(buf, buf_len, "%s", s.c_str( ));
     if ( s.size( ) + 1 > buf_len )
         return fail(FASTFFT_INVALID_ARGUMENT, "describe: buffer too small");

@@ -31,9 +31,12 @@ constexpr int kTwiddleTableN = 8192;
 // Fills tw[k] = exp(-2 pi i k / kTwiddleN) (FP64 on the GPU, rounded to FP32).
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream);
 
-// Argmax of a real [ny][pitch] image: writes {value, x, y} to result (device memory).
-hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* result_value, int* result_xy,
-                         hipStream_t stream);
+// Argmax of a real [ny][pitch] image (first maximum in row-major order).  Two
+// kernels; partial_v / partial_i hold kArgmaxPartials entries of device scratch;
+// the value goes to result_value[0] and (x, y) to result_xy[0..1].
+constexpr int kArgmaxPartials = 512;
+hipError_t    launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* partial_v, int* partial_i,
+                            float* result_value, int* result_xy, hipStream_t stream);
 
 // LDS bytes per workgroup of a line stage with sub-transform size P.
 size_t stage_lds_bytes(int P);

@@ -58,11 +58,12 @@ hipError_t prepare_launch(K kernel, int nlines, dim3& grid, dim3& block, size_t&
 
 inline int twiddle_stride(int N) { return kTwiddleN / N; }
 
-// True unless the environment variable `name` is set to a non-empty value other than "0"
-// (used to switch a specialised kernel off and fall back to the generic one).
-inline bool getenv_flag_off(const char* name) {
+// Integer value of a tuning environment variable (dflt when unset).  Read on
+// every call so that tests can switch kernels within one process; getenv is
+// cheap next to a kernel launch.
+inline int env_int(const char* name, int dflt) {
     const char* v = std::getenv(name);
-    return ! (v && v[0] && ! (v[0] == '0' && v[1] == 0));
+    return (v && v[0]) ? std::atoi(v) : dflt;
 }
 
 } // namespace fastfft_amd

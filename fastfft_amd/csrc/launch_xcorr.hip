@@ -45,8 +45,7 @@ static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const floa
 template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     constexpr int QBD = wave_qb<P, Q>( );
-    const char*   env = std::getenv("FASTFFT_XCORR_QB");
-    const int     qb  = env ? std::atoi(env) : QBD;
+    const int     qb  = env_int("FASTFFT_XCORR_QB", QBD);
     if constexpr ( Q % 4 == 0 ) {
         if ( qb == 4 )
             return go_xcorr_wave_qb<P, Q, 4, RS, TImg>(s, io, tw, st);
@@ -92,7 +91,7 @@ static bool contiguous_lines(const HostLineMap& m) { return m.es == 1 && m.n2 ==
 
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const bool wave_ok = contiguous_lines(s.im) && contiguous_lines(s.gm) && contiguous_lines(s.om) &&
-                         (io.img_ct == CpxT::c32h || s.gm.s1 % 2 == 0) && getenv_flag_off("FASTFFT_NO_WAVE_XCORR");
+                         (io.img_ct == CpxT::c32h || s.gm.s1 % 2 == 0) && ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
     if ( wave_ok ) {
         hipError_t e = hipErrorNotSupported;
         auto       pick = [&](auto pc) -> hipError_t {

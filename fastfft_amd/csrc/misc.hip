@@ -33,7 +33,7 @@ __device__ __forceinline__ Best better(Best a, Best b) {
 }
 
 constexpr int kArgmaxThreads = 256;
-constexpr int kArgmaxBlocks  = 512;
+constexpr int kArgmaxBlocks  = kArgmaxPartials;
 
 __device__ __forceinline__ Best block_reduce(Best b) {
     __shared__ float sv[kArgmaxThreads];
@@ -88,14 +88,15 @@ __global__ void __launch_bounds__(kArgmaxThreads)
     }
 }
 
-// result_value: 1 + kArgmaxBlocks floats of device scratch, result_xy: 2 + kArgmaxBlocks ints.
-hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* result_value, int* result_xy,
-                         hipStream_t stream) {
+// partial_v / partial_i: kArgmaxBlocks entries of device scratch; the result goes to
+// result_value[0] and result_xy[0..1] (device memory, may be caller buffers).
+hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* partial_v, int* partial_i,
+                         float* result_value, int* result_xy, hipStream_t stream) {
     const long n      = (long)nx * ny;
     int        blocks = (int)std::min<long>(kArgmaxBlocks, (n + kArgmaxThreads * 4 - 1) / (kArgmaxThreads * 4));
     blocks            = blocks < 1 ? 1 : blocks;
-    float* pv         = result_value + 1;
-    int*   pi         = result_xy + 2;
+    float* pv         = partial_v;
+    int*   pi         = partial_i;
     if ( rt == RealT::f32 )
         k_argmax_partial<float><<<blocks, kArgmaxThreads, 0, stream>>>(static_cast<const float*>(img), nx, ny, pitch, pv, pi);
     else

@@ -88,56 +88,60 @@ __global__ void __launch_bounds__(64)
     const TImg* gc    = img + col * ldg;
     const int   omask = ((o % N) + N) % N; // o mod N
 
-#pragma unroll 1
-    for ( int q0 = 0; q0 < Q; q0 += QB ) {
-        float2 v[QB][E];
-        float2 lw[QB]; // W_N^{t q}
-        // input twiddle W_N^{(t + 64 i) q} = W_N^{t q} * W_N^{64 i q}; the second factor is
-        // wave-uniform and comes through the scalar cache
+    static_for<Q / QB>([&](auto QI) {
+        constexpr int q0 = QI * QB;
+        // keep batches apart: interleaving them only raises register pressure
+        __builtin_amdgcn_sched_barrier(0);
+        // issue this batch's global loads first, so that they are in flight
+        // during the forward transform; the empty asm pins the address (and so
+        // the loads) to this batch instead of letting them all move to the top
+        int goff = Q * t + q0;
+        asm volatile("" : "+v"(goff));
+        float2 g[E][QB];
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            ImgLoad<QB, TImg>::run(gc + goff + (long)Q * T * i, g[i]);
+        float2 lw[QB], cq[QB]; // W_N^{t q}, W_N^{o q}
 #pragma unroll
         for ( int b = 0; b < QB; b++ ) {
-            const int q = q0 + b;
-            lw[b]       = tw[((t * q) & (N - 1)) * TWS];
-#pragma unroll
-            for ( int i = 0; i < E; i++ ) {
-                const float2 c = tw[((T * i * q) & (N - 1)) * TWS];
-                v[b][i]        = cmul(cmul(x[i], c), lw[b]);
-            }
+            lw[b] = tw[((t * (q0 + b)) & (N - 1)) * TWS];
+            cq[b] = tw[((omask * (q0 + b)) & (N - 1)) * TWS];
         }
+        float2 v[QB][E];
+        // input twiddle W_N^{(t + 64 i) q} = W_N^{t q} * W_N^{64 i q} (the second factor a constant)
+        static_for<QB>([&](auto B) {
+            constexpr int q = q0 + B;
+            static_for<E>([&](auto I) {
+                float2 u = mul_const_tw<(long)T * I * q, N, -1>(x[I]);
+                v[B][I]  = q == 0 ? u : cmul(u, lw[B]);
+            });
+        });
 #pragma unroll
         for ( int b = 0; b < QB; b++ )
             fft.template run<-1>(v[b], lds, t);
         // conj-mul with the image: Z = g * conj(X)
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int k1 = t + T * i;
-            float2    g[QB];
-            ImgLoad<QB, TImg>::run(gc + (long)Q * k1 + q0, g);
+        for ( int i = 0; i < E; i++ )
 #pragma unroll
             for ( int b = 0; b < QB; b++ )
-                v[b][i] = cmulc(g[b], v[b][i]);
-        }
+                v[b][i] = cmulc(g[i][b], v[b][i]);
 #pragma unroll
         for ( int b = 0; b < QB; b++ )
             fft.template run<+1>(v[b], lds, t);
         // accumulate: output register i' takes the IFFT value of register (i' + RS) mod E
-        // times W_N^{-(o + t + 64 i') q} = conj(W_N^{o q} W_N^{t q}) * conj(W_N^{64 i' q})
-#pragma unroll
-        for ( int b = 0; b < QB; b++ ) {
-            const int q  = q0 + b;
-            float2    cq = tw[((omask * q) & (N - 1)) * TWS]; // W_N^{o q}
-            float2    s  = cmul(lw[b], cq);                   // W_N^{(o + t) q}
-            s            = make_float2(s.x * intra_scale, -s.y * intra_scale);
-#pragma unroll
-            for ( int i = 0; i < E; i++ ) {
-                float2 c = tw[((T * i * q) & (N - 1)) * TWS];
-                c.y      = -c.y;
-                float2 u = cmul(v[b][(i + RS) % E], c);
-                acc[i].x = fmaf(u.x, s.x, fmaf(-u.y, s.y, acc[i].x));
-                acc[i].y = fmaf(u.x, s.y, fmaf(u.y, s.x, acc[i].y));
-            }
-        }
-    }
+        // times W_N^{-(o + t + 64 i') q} = conj(W_N^{o q} W_N^{t q}) * W_N^{-64 i' q}
+        static_for<QB>([&](auto B) {
+            constexpr int q = q0 + B;
+            float2        s = cmul(lw[B], cq[B]);
+            s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
+            static_for<E>([&](auto I) {
+                constexpr int src = (I + RS) % E;
+                float2        u   = mul_const_tw<(long)T * I * q, N, +1>(v[B][src]);
+                acc[I].x          = fmaf(u.x, s.x, fmaf(-u.y, s.y, acc[I].x));
+                acc[I].y          = fmaf(u.x, s.y, fmaf(u.y, s.x, acc[I].y));
+            });
+        });
+    });
     float2* oc = out + col * ldo;
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
