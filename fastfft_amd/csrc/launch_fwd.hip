@@ -1,5 +1,6 @@
 // Forward line stages: R2C x-pass (real input) and C2C y/z passes.
 #include "launch_common.hpp"
+#include "rows_wave.hpp"
 
 namespace fastfft_amd {
 
@@ -16,7 +17,46 @@ static hipError_t go_fwd(const Stage& s, const StageIO& io, const float2* tw, hi
     return hipGetLastError( );
 }
 
+// R2C rows with P in {256, 512, 1024}: one block per row, one wave per sub-transform.
+template <int P, int Q, typename TIn>
+static hipError_t go_r2c_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    k_r2c_rows_wave<P, Q, TIn><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+            static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<float2*>(io.dst), to_device(s.om), s.nout,
+            s.nlines, io.scale, tw);
+    return hipGetLastError( );
+}
+
+template <typename TIn>
+static hipError_t try_r2c_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    auto by_p = [&](auto pc) -> hipError_t {
+        constexpr int P = decltype(pc)::value;
+        switch ( s.Q ) {
+            case 1: return go_r2c_wave<P, 1, TIn>(s, io, tw, st);
+            case 2: return go_r2c_wave<P, 2, TIn>(s, io, tw, st);
+            case 4: return go_r2c_wave<P, 4, TIn>(s, io, tw, st);
+            case 8: return go_r2c_wave<P, 8, TIn>(s, io, tw, st);
+            case 16:
+                if constexpr ( P * 16 <= kTwiddleN )
+                    return go_r2c_wave<P, 16, TIn>(s, io, tw, st);
+                return hipErrorNotSupported;
+            default: return hipErrorNotSupported;
+        }
+    };
+    if ( s.P == 256 )
+        return by_p(std::integral_constant<int, 256>{ });
+    if ( s.P == 512 )
+        return by_p(std::integral_constant<int, 512>{ });
+    if ( s.P == 1024 )
+        return by_p(std::integral_constant<int, 1024>{ });
+    return hipErrorNotSupported;
+}
+
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = io.src_rt == RealT::f32 ? try_r2c_wave<float>(s, io, tw, st) : try_r2c_wave<__half>(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     return dispatch_P(s.P, [&](auto pc) -> hipError_t {
         constexpr int P = decltype(pc)::value;
         if ( io.src_real ) {

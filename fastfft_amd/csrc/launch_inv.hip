@@ -1,5 +1,6 @@
 // Inverse line stages: C2C y/z passes (output pruned) and the C2R x-pass.
 #include "launch_common.hpp"
+#include "rows_wave.hpp"
 
 namespace fastfft_amd {
 
@@ -16,7 +17,60 @@ static hipError_t go_inv(const Stage& s, const StageIO& io, const float2* tw, hi
     return hipGetLastError( );
 }
 
+// Trimmed C2R rows with P in {256, 512, 1024} and a window origin at register
+// rotation 0 or E/2: one block per row, one wave per sub-transform.
+template <int P, int Q, int RS, typename TOut>
+static hipError_t go_c2r_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+            static_cast<const float2*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
+            s.nlines, io.scale, tw);
+    return hipGetLastError( );
+}
+
+template <typename TOut>
+static hipError_t try_c2r_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( s.o % 64 != 0 )
+        return hipErrorNotSupported;
+    auto by_p = [&](auto pc) -> hipError_t {
+        constexpr int P  = decltype(pc)::value;
+        constexpr int E  = P / 64;
+        const int     rs = (((s.o / 64) % E) + E) % E;
+        if ( rs != 0 && rs != E / 2 )
+            return hipErrorNotSupported;
+        auto by_q = [&](auto qc) -> hipError_t {
+            constexpr int Q = decltype(qc)::value;
+            if constexpr ( P * Q > kTwiddleN ) {
+                return hipErrorNotSupported;
+            }
+            else {
+                return rs == 0 ? go_c2r_wave<P, Q, 0, TOut>(s, io, tw, st) : go_c2r_wave<P, Q, E / 2, TOut>(s, io, tw, st);
+            }
+        };
+        switch ( s.Q ) {
+            case 1: return by_q(std::integral_constant<int, 1>{ });
+            case 2: return by_q(std::integral_constant<int, 2>{ });
+            case 4: return by_q(std::integral_constant<int, 4>{ });
+            case 8: return by_q(std::integral_constant<int, 8>{ });
+            case 16: return by_q(std::integral_constant<int, 16>{ });
+            default: return hipErrorNotSupported;
+        }
+    };
+    if ( s.P == 256 )
+        return by_p(std::integral_constant<int, 256>{ });
+    if ( s.P == 512 )
+        return by_p(std::integral_constant<int, 512>{ });
+    if ( s.P == 1024 )
+        return by_p(std::integral_constant<int, 1024>{ });
+    return hipErrorNotSupported;
+}
+
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( s.kind == StageKind::inv_c2r && io.dst_real && ! io.src_real && io.src_ct == CpxT::c64 &&
+         ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = io.dst_rt == RealT::f32 ? try_c2r_wave<float>(s, io, tw, st) : try_c2r_wave<__half>(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     return dispatch_P(s.P, [&](auto pc) -> hipError_t {
         constexpr int P = decltype(pc)::value;
         if ( s.kind == StageKind::inv_c2r ) {

@@ -299,6 +299,36 @@ def test_xcorr_wave_kernel(tin, n, nout, origin, mode, monkeypatch):
     assert O.rel_l2(got, want) < TOL
 
 
+# the one-block-per-row x-pass kernels (R2C and trimmed C2R, one wave per sub-transform) against
+# the oracle, and the generic line kernels forced on the same shapes
+ROW_CASES = [((384, 40), (4096, 64), (512, 32), (-256, 0)), ((200, 24), (2048, 32), (256, 16), (0, 0)),
+             ((1000, 16), (4096, 32), (1024, 16), (-512, 0)), ((300, 16), (8192, 32), (512, 16), (-256, -8)),
+             ((130, 20), (1024, 32), (256, 32), (128, 0)), ((4096, 16), (4096, 16), (4096, 16), (0, 0))]
+
+
+@pytest.mark.parametrize("mode", ["default", "generic"])
+@pytest.mark.parametrize("tin,n,nout,origin", ROW_CASES)
+def test_xcorr_row_kernels(tin, n, nout, origin, mode, monkeypatch):
+    if mode == "generic":
+        monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    H.log(f"xcorr rows {tin} in {n} -> {nout} at {origin} mode={mode}")
+    rng = np.random.default_rng(tin[0] * 5 + n[0])
+    t = rng.standard_normal((tin[1], tin[0])).astype(np.float32)
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((w, n[1])) + 1j * rng.standard_normal((w, n[1]))).astype(np.complex64)
+    ft = H.make_plan((*tin, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+    tp = H.padded_real(t, tin[0]).reshape(-1)
+    a[: tp.size] = H.to_dev(tp)
+    img = H.to_dev(H.complex_to_f32(spec))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFT(a, img, out, pre_op=H.F.Scale(0.25), post_op=H.F.Scale(2.0))
+    got = H.real_window(H.to_host(out), nout[0], nout[1])
+    want = O.xcorr_2d(H.padded_real(t, tin[0]), tin[0], spec, n[0], n[1], nout[0], nout[1], *origin, pre_scale=0.25,
+                      post_scale=2.0)
+    assert O.rel_l2(got, want) < TOL
+
+
 # ------------------------------------------------------------------ the headline correlation, full size
 def test_xcorr_4096_384_trim_512_centred():
     from fastfft_amd import workloads as W
