@@ -43,7 +43,18 @@ static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const floa
 // from the measured table in the design notes; FASTFFT_XCORR_QB overrides it
 // (1, 2 or 4) for tuning runs.
 template <int P, int Q, int RS, typename TImg>
+static hipError_t go_xcorr_blk(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    k_xcorr_blk<P, Q, RS, TImg><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+            static_cast<const float2*>(io.src), s.im.s1, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
+            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw);
+    return hipGetLastError( );
+}
+
+template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    // FASTFFT_XCORR_KERNEL: 0 = one workgroup of Q waves per line (default), 1 = one wave per line
+    if ( env_int("FASTFFT_XCORR_KERNEL", 0) == 0 )
+        return go_xcorr_blk<P, Q, RS, TImg>(s, io, tw, st);
     constexpr int QBD = wave_qb<P, Q>( );
     const int     qb  = env_int("FASTFFT_XCORR_QB", QBD);
     if constexpr ( Q % 4 == 0 ) {

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(64 * Q)
         k_r2c_rows_wave(const TIn* __restrict__ in, LineMap im, int L, float2* __restrict__ out, LineMap om, int nout,
                         int nrows, float pre_scale, const float2* __restrict__ tw) {
     constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
-    __shared__ float2 lds[Q * lds_slots(P)];
+    __shared__ float2 lds[Q * WaveFFT<P>::SLOTS];
     const int q   = threadIdx.x / T;
     const int t   = threadIdx.x % T;
     const int row = xcd_row(blockIdx.x, nrows);
@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(64 * Q)
         const float2 w = tw[((n * q) & (N - 1)) * TWS];
         v[i]           = make_float2(x * w.x, x * w.y);
     }
-    fft.template run<-1>(v, lds + q * lds_slots(P), t);
+    fft.template run<-1>(v, lds + q * WaveFFT<P>::SLOTS, t);
     float2* op = out + om.base(row);
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(64 * Q)
         k_c2r_rows_wave(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o, int nrows,
                         float post_scale, const float2* __restrict__ tw) {
     constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
-    __shared__ float2 lds[Q * lds_slots(P)];
+    __shared__ float2 lds[Q * WaveFFT<P>::SLOTS];
     const int q   = threadIdx.x / T;
     const int t   = threadIdx.x % T;
     const int row = xcd_row(blockIdx.x, nrows);
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(64 * Q)
         }
         v[i] = z;
     }
-    float2* wl = lds + q * lds_slots(P);
+    float2* wl = lds + q * WaveFFT<P>::SLOTS;
     fft.template run<+1>(v, wl, t);
     // this wave's real contribution to every output sample, into its own LDS region
     float*    red   = reinterpret_cast<float*>(wl);
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(64 * Q)
         float s = 0.f;
 #pragma unroll
         for ( int w = 0; w < Q; w++ )
-            s += reinterpret_cast<const float*>(lds + w * lds_slots(P))[j];
+            s += reinterpret_cast<const float*>(lds + w * WaveFFT<P>::SLOTS)[j];
         store_real(op, j * om.es, s * post_scale);
     }
 }

@@ -112,6 +112,33 @@ __device__ __forceinline__ void wave_lds_fence( ) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---------------------------------------------------------------- LDS layout of the exchanges
+// Slot of element k in a wave's exchange buffer of P complex values: an XOR
+// swizzle within each group of 32 slots (256 bytes) that makes every Stockham
+// scatter and strided gather of the wave transforms free of LDS bank conflicts
+// (checked for 32 and 64 banks of 4 bytes); sizes without a tuned swizzle use
+// one pad slot per 32 values.
+template <int P, int E>
+struct ExchangeLayout {
+    static constexpr int SLOTS = P + P / 32;
+    __device__ __forceinline__ static int slot(int k) { return k + (k >> 5); }
+};
+template <>
+struct ExchangeLayout<256, 4> {
+    static constexpr int SLOTS = 256;
+    __device__ __forceinline__ static int slot(int k) { return k ^ ((k >> 3) & 3) ^ (((k >> 4) & 7) << 2); }
+};
+template <>
+struct ExchangeLayout<512, 8> {
+    static constexpr int SLOTS = 512;
+    __device__ __forceinline__ static int slot(int k) { return k ^ ((k >> 3) & 3) ^ (((k >> 5) & 7) << 2); }
+};
+template <>
+struct ExchangeLayout<1024, 16> {
+    static constexpr int SLOTS = 1024;
+    __device__ __forceinline__ static int slot(int k) { return k ^ ((k >> 4) & 3) ^ (((k >> 6) & 7) << 2); }
+};
+
 // ---------------------------------------------------------------- wave Stockham
 // Radix of pass p (stride Ns = E^p ...): E for all passes but a smaller last one.
 template <int P, int E>
@@ -125,6 +152,8 @@ struct WaveFFT {
     static constexpr int T      = 64;
     static constexpr int E      = P / T;
     static constexpr int NPASS  = WavePasses<P, E>::N_PASS;
+    using Layout                = ExchangeLayout<P, E>;
+    static constexpr int SLOTS  = Layout::SLOTS; // LDS complex values per transform
     static_assert(P % 64 == 0 && E >= 2 && E <= 16, "wave FFT needs 2..16 values per lane");
     // twiddles of passes 1..NPASS-1: per pass at most B*(R-1) < E values
     float2 w[NPASS > 1 ? NPASS - 1 : 1][E];
@@ -149,57 +178,71 @@ struct WaveFFT {
 
     __device__ __forceinline__ void init(int t, const float2* __restrict__ tw) { init_pass<1, 0>(t, tw); }
 
-    template <int Ns, int PI, int DIR>
-    __device__ __forceinline__ void pass(float2 (&v)[E], float2* __restrict__ lds, int t) const {
+    // One Stockham pass applied to NB independent transforms held by the wave;
+    // each transform exchanges through its own LDS region of SLOTS values, so the
+    // NB exchanges of a pass share one LDS round trip.
+    template <int Ns, int PI, int DIR, int NB>
+    __device__ __forceinline__ void pass(float2 (&v)[NB][E], float2* __restrict__ lds, int t) const {
         if constexpr ( Ns < P ) {
             constexpr int  R    = (P / Ns) < E ? (P / Ns) : E;
             constexpr int  B    = E / R;
             constexpr bool LAST = (Ns * R == P);
 #pragma unroll
-            for ( int m = 0; m < B; m++ ) {
-                float2 u[R];
+            for ( int nb = 0; nb < NB; nb++ ) {
 #pragma unroll
-                for ( int r = 0; r < R; r++ )
-                    u[r] = v[m + B * r];
-                if constexpr ( Ns > 1 ) {
+                for ( int m = 0; m < B; m++ ) {
+                    float2 u[R];
 #pragma unroll
-                    for ( int r = 1; r < R; r++ ) {
-                        float2 tw = w[PI - 1][m * (R - 1) + r - 1];
-                        if constexpr ( DIR > 0 )
-                            tw.y = -tw.y;
-                        u[r] = cmul(u[r], tw);
+                    for ( int r = 0; r < R; r++ )
+                        u[r] = v[nb][m + B * r];
+                    if constexpr ( Ns > 1 ) {
+#pragma unroll
+                        for ( int r = 1; r < R; r++ ) {
+                            float2 tw = w[PI - 1][m * (R - 1) + r - 1];
+                            if constexpr ( DIR > 0 )
+                                tw.y = -tw.y;
+                            u[r] = cmul(u[r], tw);
+                        }
                     }
-                }
-                SmallDFT<R, DIR>::run(u);
-                if constexpr ( LAST ) {
+                    SmallDFT<R, DIR>::run(u);
+                    if constexpr ( LAST ) {
 #pragma unroll
-                    for ( int r = 0; r < R; r++ )
-                        v[m + B * r] = u[r];
-                }
-                else {
-                    const int b    = t + T * m;
-                    const int j    = b & (Ns - 1);
-                    const int base = (b - j) * R + j;
+                        for ( int r = 0; r < R; r++ )
+                            v[nb][m + B * r] = u[r];
+                    }
+                    else {
+                        const int b    = t + T * m;
+                        const int j    = b & (Ns - 1);
+                        const int base = (b - j) * R + j;
 #pragma unroll
-                    for ( int r = 0; r < R; r++ )
-                        lds[lds_slot(base + r * Ns)] = u[r];
+                        for ( int r = 0; r < R; r++ )
+                            lds[nb * SLOTS + Layout::slot(base + r * Ns)] = u[r];
+                    }
                 }
             }
             if constexpr ( ! LAST ) {
                 wave_lds_fence( );
 #pragma unroll
-                for ( int i = 0; i < E; i++ )
-                    v[i] = lds[lds_slot(t + T * i)];
+                for ( int nb = 0; nb < NB; nb++ )
+#pragma unroll
+                    for ( int i = 0; i < E; i++ )
+                        v[nb][i] = lds[nb * SLOTS + Layout::slot(t + T * i)];
                 wave_lds_fence( );
             }
-            pass<Ns * R, PI + 1, DIR>(v, lds, t);
+            pass<Ns * R, PI + 1, DIR, NB>(v, lds, t);
         }
     }
 
-    // In-place transform of one wave-held P-point sequence.  lds: lds_slots(P) values.
+    // NB transforms in lockstep; lds holds NB * SLOTS values.
+    template <int DIR, int NB>
+    __device__ __forceinline__ void run_batch(float2 (&v)[NB][E], float2* __restrict__ lds, int t) const {
+        pass<1, 0, DIR, NB>(v, lds, t);
+    }
+
+    // In-place transform of one wave-held P-point sequence.  lds: SLOTS values.
     template <int DIR>
     __device__ __forceinline__ void run(float2 (&v)[E], float2* __restrict__ lds, int t) const {
-        pass<1, 0, DIR>(v, lds, t);
+        pass<1, 0, DIR, 1>(reinterpret_cast<float2(&)[1][E]>(v), lds, t);
     }
 };
 

@@ -275,11 +275,12 @@ WAVE_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((48, 384), (64, 40
               ((24, 1000), (32, 4096), (16, 1024), (0, -512)), ((16, 130), (32, 1024), (16, 256), (0, 128))]
 
 
-@pytest.mark.parametrize("mode", ["default", "qb1", "qb2", "qb4", "generic"])
+@pytest.mark.parametrize("mode", ["default", "wave_qb1", "wave_qb2", "wave_qb4", "generic"])
 @pytest.mark.parametrize("tin,n,nout,origin", WAVE_CASES)
 def test_xcorr_wave_kernel(tin, n, nout, origin, mode, monkeypatch):
-    env = {"default": {}, "qb1": {"FASTFFT_XCORR_QB": "1"}, "qb2": {"FASTFFT_XCORR_QB": "2"},
-           "qb4": {"FASTFFT_XCORR_QB": "4"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
+    wave = {"FASTFFT_XCORR_KERNEL": "1"}
+    env = {"default": {}, "wave_qb1": {**wave, "FASTFFT_XCORR_QB": "1"}, "wave_qb2": {**wave, "FASTFFT_XCORR_QB": "2"},
+           "wave_qb4": {**wave, "FASTFFT_XCORR_QB": "4"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     H.log(f"xcorr wave {tin} in {n} -> {nout} at {origin} mode={mode}")
