@@ -56,6 +56,12 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int idx
     return w;
 }
 
+// Multi-resolution part of the table, after the kTwiddleN base entries:
+// multi_twiddles(tw)[n + m] = W_n^m = exp(-2 pi i m / n) for every power of two
+// n <= kTwiddleN and m < n, so that the twiddles of any one transform size are
+// contiguous (coalesced loads).
+__device__ __forceinline__ const float2* multi_twiddles(const float2* tw) { return tw + kTwiddleN; }
+
 // ---------------------------------------------------------------- type conversion
 template <typename T>
 __device__ __forceinline__ float load_real(const T* p, long i);

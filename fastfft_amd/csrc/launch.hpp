@@ -25,10 +25,11 @@ hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipSt
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 
-// Length of the twiddle table (the largest supported line transform).
-constexpr int kTwiddleTableN = 8192;
+// Entries of the twiddle table: W_8192^k for k < 8192, then the
+// multi-resolution part (W_n^m at 8192 + n + m for every power of two n <= 8192).
+constexpr int kTwiddleTableN = 3 * 8192;
 
-// Fills tw[k] = exp(-2 pi i k / kTwiddleN) (FP64 on the GPU, rounded to FP32).
+// Fills the twiddle table (FP64 on the GPU, rounded to FP32).
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream);
 
 // Argmax of a real [ny][pitch] image (first maximum in row-major order).  Two

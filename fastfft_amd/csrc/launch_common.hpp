@@ -9,7 +9,7 @@
 
 namespace fastfft_amd {
 
-static_assert(kTwiddleTableN == kTwiddleN, "host and device twiddle table lengths differ");
+static_assert(kTwiddleTableN == kTwiddleN + 2 * kTwiddleN, "host and device twiddle table lengths differ");
 
 inline LineMap to_device(const HostLineMap& m) {
     LineMap d;

@@ -2,9 +2,13 @@
 //
 // Contiguous lines (the 2-D second axis) with P in {256, 512, 1024}, a window
 // origin that is a multiple of 64 placed at register rotation 0 or E/2, and
-// N = P*Q <= 8192 run the one-wave-per-line kernel of xcorr_wave.hpp; every
-// other shape runs the generic line kernel.
+// N = P*Q <= 8192 run the workgroup-per-column kernel of xcorr_col.hpp (or,
+// with FASTFFT_XCORR_KERNEL=1, the wave-per-column kernel of xcorr_wave.hpp);
+// every other shape runs the generic line kernel.
+#include <cstdint>
+
 #include "launch_common.hpp"
+#include "xcorr_col.hpp"
 #include "xcorr_wave.hpp"
 
 namespace fastfft_amd {
@@ -23,14 +27,6 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
     return hipGetLastError( );
 }
 
-// q batch held in registers: 32 complex values per lane
-template <int P, int Q>
-constexpr int wave_qb( ) {
-    constexpr int E  = P / 64;
-    constexpr int qb = 16 / E;
-    return Q < qb ? Q : qb;
-}
-
 template <int P, int Q, int QB, int RS, typename TImg>
 static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     k_xcorr_wave<P, Q, QB, RS, TImg><<<dim3((unsigned)s.nlines), dim3(64), 0, st>>>(
@@ -39,35 +35,31 @@ static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const floa
     return hipGetLastError( );
 }
 
-// Batch size of q sub-transforms held in registers at once.  The default comes
-// from the measured table in the design notes; FASTFFT_XCORR_QB overrides it
-// (1, 2 or 4) for tuning runs.
+// One workgroup per column; columns 0 and nlines-1 share workgroup 0 when they
+// are the DC and Nyquist columns of a real transform (FASTFFT_XCORR_PACK=0
+// gives them a workgroup each, for tests).
 template <int P, int Q, int RS, typename TImg>
-static hipError_t go_xcorr_blk(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    k_xcorr_blk<P, Q, RS, TImg><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const int pack   = s.real_end_lines && s.nlines >= 2 && env_int("FASTFFT_XCORR_PACK", 1) ? 1 : 0;
+    const int blocks = s.nlines - pack;
+    k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks), dim3(64 * Q), 0, st>>>(
             static_cast<const float2*>(io.src), s.im.s1, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
-            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw);
+            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw, s.nlines - 1, pack);
     return hipGetLastError( );
 }
 
 template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    // FASTFFT_XCORR_KERNEL: 0 = one workgroup of Q waves per line (default), 1 = one wave per line
-    if ( env_int("FASTFFT_XCORR_KERNEL", 0) == 0 )
-        return go_xcorr_blk<P, Q, RS, TImg>(s, io, tw, st);
-    constexpr int QBD = wave_qb<P, Q>( );
-    const int     qb  = env_int("FASTFFT_XCORR_QB", QBD);
-    if constexpr ( Q % 4 == 0 ) {
-        if ( qb == 4 )
-            return go_xcorr_wave_qb<P, Q, 4, RS, TImg>(s, io, tw, st);
-    }
-    if constexpr ( Q % 2 == 0 ) {
-        if ( qb == 2 )
+    // FASTFFT_XCORR_KERNEL: 0 = one workgroup of Q waves per column (default),
+    // 1 = one wave per column (P = 512, Q = 8 only; kept for comparison)
+    if constexpr ( P == 512 && Q == 8 ) {
+        if ( env_int("FASTFFT_XCORR_KERNEL", 0) == 1 ) {
+            if ( env_int("FASTFFT_XCORR_QB", 2) == 1 )
+                return go_xcorr_wave_qb<P, Q, 1, RS, TImg>(s, io, tw, st);
             return go_xcorr_wave_qb<P, Q, 2, RS, TImg>(s, io, tw, st);
+        }
     }
-    if ( qb == 1 )
-        return go_xcorr_wave_qb<P, Q, 1, RS, TImg>(s, io, tw, st);
-    return go_xcorr_wave_qb<P, Q, QBD, RS, TImg>(s, io, tw, st);
+    return go_xcorr_col<P, Q, RS, TImg>(s, io, tw, st);
 }
 
 template <int P, typename TImg>
@@ -101,8 +93,11 @@ static hipError_t try_xcorr_wave(const Stage& s, const StageIO& io, const float2
 static bool contiguous_lines(const HostLineMap& m) { return m.es == 1 && m.n2 == 1 && m.s2 == 0; }
 
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    // 16-byte image loads: aligned image base and column starts
+    const int  per     = io.img_ct == CpxT::c32h ? 4 : 2;
     const bool wave_ok = contiguous_lines(s.im) && contiguous_lines(s.gm) && contiguous_lines(s.om) &&
-                         (io.img_ct == CpxT::c32h || s.gm.s1 % 2 == 0) && ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
+                         reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 && s.gm.s1 % per == 0 &&
+                         ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
     if ( wave_ok ) {
         hipError_t e = hipErrorNotSupported;
         auto       pick = [&](auto pc) -> hipError_t {

@@ -3,18 +3,32 @@
 
 namespace fastfft_amd {
 
+// tw[k] = W_8192^k for k < 8192; tw[8192 + n + m] = W_n^m for powers of two n
+// and m < n (entry 8192 + 0 is unused and holds 1).
 __global__ void k_twiddle_init(float2* __restrict__ tw) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( k < kTwiddleN ) {
-        double s, c;
-        // exp(-2 pi i k / NT): angle in units of pi is 2k/NT, exact in binary
-        sincospi(2.0 * (double)k / (double)kTwiddleN, &s, &c);
-        tw[k] = make_float2((float)c, (float)(-s));
+    if ( k >= kTwiddleTableN )
+        return;
+    long m = k, n = kTwiddleN;
+    if ( k >= kTwiddleN ) {
+        const int idx = k - kTwiddleN;
+        n             = 1;
+        while ( 2 * n <= idx )
+            n *= 2;
+        m = idx - n;
+        if ( idx == 0 ) {
+            m = 0;
+            n = 1;
+        }
     }
+    double s, c;
+    // exp(-2 pi i m / n): the angle in units of pi is 2m/n, exact in binary
+    sincospi(2.0 * (double)m / (double)n, &s, &c);
+    tw[k] = make_float2((float)c, (float)(-s));
 }
 
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream) {
-    k_twiddle_init<<<dim3((kTwiddleN + 255) / 256), dim3(256), 0, stream>>>(tw);
+    k_twiddle_init<<<dim3((kTwiddleTableN + 255) / 256), dim3(256), 0, stream>>>(tw);
     return hipGetLastError( );
 }
 

@@ -49,6 +49,10 @@ struct Stage {
     Buf         src = Buf::user_in, dst = Buf::user_out;
     Elem        src_elem = Elem::cpx_f32, dst_elem = Elem::cpx_f32;
     HostLineMap im, om, gm; // input, output and (xcorr) image maps
+    // xcorr over the kx columns of a real 2-D transform with an even x size:
+    // lines 0 and nlines-1 are the DC and Nyquist columns, whose template
+    // values are real and of whose outputs the C2R x-pass uses only the real part
+    bool real_end_lines = false;
 };
 
 struct Dims {

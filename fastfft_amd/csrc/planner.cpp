@@ -2,6 +2,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <sstream>
 
 #include "../../include/fastfft/fastfft.h"
@@ -145,6 +146,16 @@ int pick_sub_size(int N, int L, int M) {
     return std::min(P, N);
 }
 
+// Leading dimension (complex elements) of a workspace array of lines holding n
+// values each.  The x-passes access these arrays across lines (one element per
+// line); FASTFFT_WS_PAD adds elements per line to move those accesses off a
+// single memory channel.
+static long ws_ld(int n) {
+    const char* v   = std::getenv("FASTFFT_WS_PAD");
+    const long  pad = v ? std::max(0L, std::atol(v)) : 0L;
+    return n + pad;
+}
+
 static HostLineMap lm(long s1, long s2, long es, int n2) {
     HostLineMap m;
     m.s1 = s1;
@@ -187,7 +198,7 @@ static PlanError fwd_stages_2d(const PlanState& p, std::vector<Stage>& st, Buf f
     s1.im       = lm(pitch, 0, 1, 1);
     s1.dst      = Buf::ws_a;
     s1.dst_elem = Elem::cpx_f32;
-    s1.om       = lm(1, 0, in.y, 1);
+    s1.om       = lm(1, 0, ws_ld(in.y), 1);
     st.push_back(s1);
     // y-pass: each of the out.w columns A[kx][0..in.y) -> C[kx][ky]
     const int Ny = std::max(in.y, out.y);
@@ -196,7 +207,7 @@ static PlanError fwd_stages_2d(const PlanState& p, std::vector<Stage>& st, Buf f
                               Ny, in.y, 0, 0, out.y, out.w);
     s2.src      = Buf::ws_a;
     s2.src_elem = Elem::cpx_f32;
-    s2.im       = lm(in.y, 0, 1, 1);
+    s2.im       = lm(ws_ld(in.y), 0, 1, 1);
     s2.dst      = final_dst;
     s2.dst_elem = Elem::cpx_input_type;
     s2.om       = lm(out.y, 0, 1, 1);
@@ -264,14 +275,14 @@ static PlanError inv_stages_2d(const PlanState& p, std::vector<Stage>& st, Buf s
     s1.im       = lm(in.y, 0, 1, 1);
     s1.dst      = Buf::ws_b;
     s1.dst_elem = Elem::cpx_f32;
-    s1.om       = lm(out.y, 0, 1, 1);
+    s1.om       = lm(ws_ld(out.y), 0, 1, 1);
     st.push_back(s1);
     // x-pass C2R on the out.y rows -> out[j][pitch]
     Stage s2    = make_stage(StageKind::inv_c2r, dec ? "c2r_decrease_XY" : "c2r_none_XY", in.x, 0, out.x,
                              axis_origin(p, 0, in.x, out.x), 0, out.y);
     s2.src      = Buf::ws_b;
     s2.src_elem = Elem::cpx_f32;
-    s2.im       = lm(1, 0, out.y, 1);
+    s2.im       = lm(1, 0, ws_ld(out.y), 1);
     s2.dst      = Buf::user_out;
     s2.dst_elem = Elem::real_input_type;
     s2.om       = lm(2 * out.w, 0, 1, 1);
@@ -344,7 +355,7 @@ static PlanError xcorr_stages_2d(const PlanState& p, std::vector<Stage>& st) {
     s1.im             = lm(pitch, 0, 1, 1);
     s1.dst            = Buf::ws_a;
     s1.dst_elem       = Elem::cpx_f32;
-    s1.om             = lm(1, 0, in.y, 1);
+    s1.om             = lm(1, 0, ws_ld(in.y), 1);
     st.push_back(s1);
     const bool dec = p.inv_change == SizeChange::decrease;
     Stage      s2  = make_stage(StageKind::xcorr,
@@ -353,17 +364,18 @@ static PlanError xcorr_stages_2d(const PlanState& p, std::vector<Stage>& st) {
                                 mid.y, in.y, out.y, axis_origin(p, 1, mid.y, out.y), 0, mid.w);
     s2.src         = Buf::ws_a;
     s2.src_elem    = Elem::cpx_f32;
-    s2.im          = lm(in.y, 0, 1, 1);
+    s2.im          = lm(ws_ld(in.y), 0, 1, 1);
     s2.gm          = lm(mid.y, 0, 1, 1);
     s2.dst         = Buf::ws_b;
     s2.dst_elem    = Elem::cpx_f32;
-    s2.om          = lm(out.y, 0, 1, 1);
+    s2.om          = lm(ws_ld(out.y), 0, 1, 1);
+    s2.real_end_lines = mid.x % 2 == 0 && mid.w >= 2;
     st.push_back(s2);
     Stage s3    = make_stage(StageKind::inv_c2r, dec ? "c2r_decrease_XY" : "c2r_none_XY", mid.x, 0, out.x,
                              axis_origin(p, 0, mid.x, out.x), 0, out.y);
     s3.src      = Buf::ws_b;
     s3.src_elem = Elem::cpx_f32;
-    s3.im       = lm(1, 0, out.y, 1);
+    s3.im       = lm(1, 0, ws_ld(out.y), 1);
     s3.dst      = Buf::user_out;
     s3.dst_elem = Elem::real_input_type;
     s3.om       = lm(2 * out.w, 0, 1, 1);

@@ -41,7 +41,7 @@ constexpr double cos_series(double x) {
 
 // cos and sin of 2*pi*k/n with exact reduction to [0, pi/4] (n a multiple of 8 or any n).
 struct CS {
-    double c, s;
+    double c = 0.0, s = 0.0;
 };
 
 constexpr CS cis(long k, long n) {
@@ -54,7 +54,7 @@ constexpr CS cis(long k, long n) {
     long  quad = (4 * k) / n;
     // remainder angle in radians, computed from the exact rational remainder
     double rem = 2.0 * kPi * (double)(4 * k - quad * n) / (4.0 * (double)n);
-    double c, s;
+    double c = 0.0, s = 0.0;
     if ( q8 % 2 == 0 ) {
         c = cos_series(rem);
         s = sin_series(rem);
@@ -158,8 +158,9 @@ struct WaveFFT {
     // twiddles of passes 1..NPASS-1: per pass at most B*(R-1) < E values
     float2 w[NPASS > 1 ? NPASS - 1 : 1][E];
 
-    template <int Ns, int PI>
-    __device__ __forceinline__ void init_pass(int t, const float2* __restrict__ tw) {
+    // tab[k] = W_TN^k for k < TN (TN a multiple of P)
+    template <int Ns, int PI, int TN>
+    __device__ __forceinline__ void init_pass(int t, const float2* __restrict__ tab) {
         if constexpr ( Ns < P ) {
             constexpr int R = (P / Ns) < E ? (P / Ns) : E;
             constexpr int B = E / R;
@@ -169,14 +170,18 @@ struct WaveFFT {
                     const int j = (t + T * m) & (Ns - 1);
 #pragma unroll
                     for ( int r = 1; r < R; r++ )
-                        w[PI - 1][m * (R - 1) + r - 1] = tw[j * r * (kTwiddleN / (Ns * R))];
+                        w[PI - 1][m * (R - 1) + r - 1] = tab[j * r * (TN / (Ns * R))];
                 }
             }
-            init_pass<Ns * R, PI + 1>(t, tw);
+            init_pass<Ns * R, PI + 1, TN>(t, tab);
         }
     }
 
-    __device__ __forceinline__ void init(int t, const float2* __restrict__ tw) { init_pass<1, 0>(t, tw); }
+    // Stockham twiddles from the global table (W_kTwiddleN^k)
+    __device__ __forceinline__ void init(int t, const float2* __restrict__ tw) { init_pass<1, 0, kTwiddleN>(t, tw); }
+
+    // Stockham twiddles from a table of W_P^k, k < P (for example a copy in LDS)
+    __device__ __forceinline__ void init_p(int t, const float2* tab) { init_pass<1, 0, P>(t, tab); }
 
     // One Stockham pass applied to NB independent transforms held by the wave;
     // each transform exchanges through its own LDS region of SLOTS values, so the

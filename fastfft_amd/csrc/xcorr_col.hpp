@@ -1,0 +1,209 @@
+// Fused correlation pass with one workgroup of Q wavefronts per column (the
+// default hot kernel of FwdImageInvFFT's second axis; replaces the reference's
+// block_fft_kernel_C2C_FWD_INCREASE_OP_INV_NONE / _FWD_NONE_INV_DECREASE_ConjMul,
+// src/fastfft/FastFFT.cu:20-56, :1295-1344, for the padded-and-trimmed case).
+//
+// For a column of full length N = P*Q, wave q of the workgroup computes
+//   X[Q k' + q] = FFT_P( a[n] W_N^{n q} )[k'],                       n < L
+//   Z[Q k' + q] = img[Q k' + q] * conj(X[Q k' + q]) * s
+//   U_q         = IFFT_P( Z[Q k' + q] )
+// and leaves its share W_N^{-(o+j) q} U_q[(o + j) mod P] of the M outputs in
+// LDS; the workgroup sums the Q shares.  The image column is first copied into
+// LDS with coalesced 16-byte loads, so that the decimated reads of the waves
+// never go to memory.
+//
+// DC and Nyquist columns.  Over the columns kx = 0 .. NX/2 of a real 2-D
+// transform with even NX, the template values of columns 0 and NX/2 are real
+// and the C2R x-pass uses only the real part of their outputs.  Workgroup 0
+// computes both columns with one transform of z = a0 + i a1 (a0, a1 the real
+// template columns): with Z = FFT(z) and H_c[k] = (I_c[k] + conj I_c[-k]) / 2
+// the Hermitian part of image column c, the real outputs are
+//   Re IFFT(I_0 conj FFT(a0)) + i Re IFFT(I_1 conj FFT(a1))
+//       = IFFT( (conj Z[k] (H_0 - H_1)[k] + Z[-k] (H_0 + H_1)[k]) / 2 ),
+// so NX/2 workgroups cover the NX/2 + 1 columns and every workgroup has the
+// same amount of work.  Those two columns are stored with a zero imaginary part.
+#pragma once
+
+#include "wave_fft.hpp"
+
+namespace fastfft_amd {
+
+// Slot of image element k = Q k' + q in the staging buffer: the Q decimated
+// sequences one after another, each XOR-swizzled inside blocks of 64 slots, so
+// that the coalesced staging writes (consecutive k, two or four per lane) and
+// the reads of wave q (consecutive k') are both free of LDS bank conflicts.
+template <int P, int Q>
+__device__ __forceinline__ int img_slot(int k) {
+    const int q = k % Q, kp = k / Q;
+    return q * P + (kp ^ (((q >> 1) * (128 / Q)) & 63));
+}
+
+// 16-byte loads of an image column: PER complex values per load.
+template <typename TImg>
+struct ImgStage;
+
+template <>
+struct ImgStage<float2> {
+    static constexpr int PER = 2;
+    __device__ __forceinline__ static void load(const float2* __restrict__ p, int m, float2 (&v)[PER]) {
+        const float4 f = reinterpret_cast<const float4*>(p)[m];
+        v[0]           = make_float2(f.x, f.y);
+        v[1]           = make_float2(f.z, f.w);
+    }
+};
+
+template <>
+struct ImgStage<__half2> {
+    static constexpr int PER = 4;
+    __device__ __forceinline__ static void load(const __half2* __restrict__ p, int m, float2 (&v)[PER]) {
+        const uint4    u = reinterpret_cast<const uint4*>(p)[m];
+        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for ( int e = 0; e < 4; e++ ) {
+            __half2 h;
+            __builtin_memcpy(&h, &w[e], sizeof(h));
+            v[e] = __half22float2(h);
+        }
+    }
+};
+
+// Grid: nlines - 1 workgroups when pack_ends (columns 0 and last_line share
+// workgroup 0), else nlines.  a, img, out: column c starts at c*lda, c*ldg,
+// c*ldo with unit element stride; img and img + ldg must be 16-byte aligned.
+//
+// Every table a wave needs is copied into LDS once per workgroup with
+// coalesced loads (W_P^k for the Stockham passes, W_N^k for k < 64 Q for the
+// per-lane input and output twiddles, the template column), and the
+// wave-uniform twiddles W_N^{64 i q} are scalar loads, so the only vector
+// memory traffic is the coalesced column data.
+template <int P, int Q, int RS, typename TImg>
+__global__ void __launch_bounds__(64 * Q)
+        k_xcorr_col(const float2* __restrict__ a, long lda, int L, const TImg* __restrict__ img, long ldg,
+                    float2* __restrict__ out, long ldo, int M, int o, float intra_scale, const float2* __restrict__ tw,
+                    int last_line, int pack_ends) {
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, NW = T * Q;
+    constexpr int PER = ImgStage<TImg>::PER;
+    using FFT         = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "the staged column must fit the exchange buffers");
+    static_assert(E % PER == 0, "whole 16-byte loads per thread");
+    static_assert(RS >= 0 && RS < E, "register rotation out of range");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];  // W_P^k
+    __shared__ float2 s_twn[NW]; // W_N^k, k < 64 Q (covers t q)
+    __shared__ float2 s_tpl[P];  // template column, first L values
+    const int      q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int      t    = threadIdx.x % T;
+    const long     col  = blockIdx.x;
+    const bool     ends = pack_ends && blockIdx.x == 0;
+    const float2*  twm  = multi_twiddles(tw);
+
+    // ---- stage: twiddle tables, template column, image column
+    for ( int k = threadIdx.x; k < P; k += NT )
+        s_twp[k] = twm[P + k];
+    for ( int k = threadIdx.x; k < NW; k += NT )
+        s_twn[k] = twm[N + k];
+    {
+        const float2* ac = a + col * lda;
+        const float2* a1 = a + (long)last_line * lda;
+        for ( int n = threadIdx.x; n < L; n += NT )
+            s_tpl[n] = ends ? make_float2(ac[n].x, a1[n].x) : ac[n];
+    }
+    if ( ! ends ) {
+        const TImg* gc = img + col * ldg;
+#pragma unroll
+        for ( int r = 0; r < E / PER; r++ ) {
+            const int m = threadIdx.x + NT * r;
+            float2    f[PER];
+            ImgStage<TImg>::load(gc, m, f);
+#pragma unroll
+            for ( int e = 0; e < PER; e++ )
+                lds[img_slot<P, Q>(PER * m + e)] = f[e];
+        }
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_p(t, s_twp);
+    // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
+    const float2 lw = s_twn[t * q];
+    float2       v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int    n = t + T * i;
+        const float2 x = n < L ? s_tpl[n] : make_float2(0.f, 0.f);
+        v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
+    }
+    float2 g[E];
+    if ( ! ends ) {
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            g[i] = lds[img_slot<P, Q>(Q * (t + T * i) + q)];
+    }
+    __syncthreads( );
+    float2* wl = lds + q * FFT::SLOTS;
+    fft.template run<-1>(v, wl, t);
+    if ( ! ends ) {
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            v[i] = cmulc(g[i], v[i]); // img * conj(X)
+    }
+    else {
+        // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
+        // k'' = (P - k') mod P when q = 0, P - 1 - k' otherwise
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            wl[FFT::Layout::slot(t + T * i)] = v[i];
+        __syncthreads( );
+        const int qp = (Q - q) % Q;
+        float2    zm[E];
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int kp  = t + T * i;
+            const int kpp = q == 0 ? (P - kp) & (P - 1) : P - 1 - kp;
+            zm[i]         = lds[qp * FFT::SLOTS + FFT::Layout::slot(kpp)];
+        }
+        __syncthreads( );
+        const TImg* g0 = img;
+        const TImg* g1 = img + (long)last_line * ldg;
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int    k  = Q * (t + T * i) + q;
+            const int    km = (N - k) & (N - 1);
+            const float2 i0 = load_cpx(g0, k), i0m = load_cpx(g0, km);
+            const float2 i1 = load_cpx(g1, k), i1m = load_cpx(g1, km);
+            const float2 h0 = make_float2(i0.x + i0m.x, i0.y - i0m.y); // 2 H_0[k]
+            const float2 h1 = make_float2(i1.x + i1m.x, i1.y - i1m.y); // 2 H_1[k]
+            const float2 d  = csub(h0, h1), sm = cadd(h0, h1);
+            const float2 r  = cadd(cmul(make_float2(v[i].x, -v[i].y), d), cmul(zm[i], sm));
+            v[i]            = make_float2(0.25f * r.x, 0.25f * r.y);
+        }
+    }
+    fft.template run<+1>(v, wl, t);
+    // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
+    // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+    const int omask = ((o % N) + N) % N;
+    float2    s     = cmul(lw, twm[N + ((omask * q) & (N - 1))]);
+    s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        float2 c = twm[N + ((T * i * q) & (N - 1))];
+        c.y      = -c.y;
+        wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
+    }
+    __syncthreads( );
+    for ( int j = threadIdx.x; j < M; j += NT ) {
+        float2    acc = make_float2(0.f, 0.f);
+        const int sj  = FFT::Layout::slot(j);
+#pragma unroll
+        for ( int w = 0; w < Q; w++ )
+            acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
+        if ( ! ends ) {
+            out[col * ldo + j] = acc;
+        }
+        else {
+            out[j]                         = make_float2(acc.x, 0.f);
+            out[(long)last_line * ldo + j] = make_float2(acc.y, 0.f);
+        }
+    }
+}
+
+} // namespace fastfft_amd

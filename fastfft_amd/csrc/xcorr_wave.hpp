@@ -148,73 +148,3 @@ __global__ void __launch_bounds__(64)
 }
 
 } // namespace fastfft_amd
-
-namespace fastfft_amd {
-
-// Block variant of the fused correlation pass: one workgroup per line, one
-// wavefront per sub-transform q (Q waves), E = P/64 values per lane.  Each wave
-// transforms its decimated sequence, multiplies by the image, transforms back and
-// leaves its weighted contribution to the M outputs in its own LDS region; the
-// workgroup then sums the Q regions.  Few live registers per wave (one
-// sub-transform) keep 4-6 waves per SIMD resident, which hides the LDS and
-// memory latency of the exchanges.
-template <int P, int Q, int RS, typename TImg>
-__global__ void __launch_bounds__(64 * Q)
-        k_xcorr_blk(const float2* __restrict__ a, long lda, int L, const TImg* __restrict__ img, long ldg,
-                    float2* __restrict__ out, long ldo, int M, int o, float intra_scale, const float2* __restrict__ tw) {
-    constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
-    using FFT       = WaveFFT<P>;
-    __shared__ float2 lds[Q * FFT::SLOTS];
-    const int  q   = threadIdx.x / T;
-    const int  t   = threadIdx.x % T;
-    const long col = blockIdx.x;
-    FFT        fft;
-    fft.init(t, tw);
-
-    // image values of this sub-transform, issued first: g[i] = img[Q (t + 64 i) + q]
-    const TImg* gc = img + col * ldg + q;
-    float2      g[E];
-#pragma unroll
-    for ( int i = 0; i < E; i++ )
-        g[i] = load_cpx(gc, (long)Q * (t + T * i));
-    // input: a[n] W_N^{n q}, n = t + 64 i
-    const float2* ac = a + col * lda;
-    const float2  lw = tw[((t * q) & (N - 1)) * TWS]; // W_N^{t q}
-    float2        v[E];
-#pragma unroll
-    for ( int i = 0; i < E; i++ ) {
-        const int    n = t + T * i;
-        const float2 x = n < L ? ac[n] : make_float2(0.f, 0.f);
-        const float2 c = tw[((T * i * q) & (N - 1)) * TWS]; // W_N^{64 i q}, wave-uniform
-        v[i]           = cmul(cmul(x, c), lw);
-    }
-    float2* wl = lds + q * FFT::SLOTS;
-    fft.template run<-1>(v, wl, t);
-#pragma unroll
-    for ( int i = 0; i < E; i++ )
-        v[i] = cmulc(g[i], v[i]);
-    fft.template run<+1>(v, wl, t);
-    // contribution e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] to output j = t + 64 i
-    const int omask = ((o % N) + N) % N;
-    float2    s     = cmul(lw, tw[((omask * q) & (N - 1)) * TWS]); // W_N^{(o + t) q}
-    s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
-#pragma unroll
-    for ( int i = 0; i < E; i++ ) {
-        float2 c = tw[((T * i * q) & (N - 1)) * TWS];
-        c.y      = -c.y;
-        float2 u = cmul(cmul(v[(i + RS) % E], c), s);
-        wl[FFT::Layout::slot(t + T * i)] = u;
-    }
-    __syncthreads( );
-    float2* oc = out + col * ldo;
-    for ( int j = threadIdx.x; j < M; j += T * Q ) {
-        float2 acc = make_float2(0.f, 0.f);
-        const int sj = FFT::Layout::slot(j);
-#pragma unroll
-        for ( int w = 0; w < Q; w++ )
-            acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
-        oc[j] = acc;
-    }
-}
-
-} // namespace fastfft_amd
