@@ -1,4 +1,6 @@
 // Forward line stages: R2C x-pass (real input) and C2C y/z passes.
+#include <cstdint>
+
 #include "launch_common.hpp"
 #include "rows_wave.hpp"
 
@@ -52,7 +54,10 @@ static hipError_t try_r2c_wave(const Stage& s, const StageIO& io, const float2* 
 }
 
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+    // a unit-stride output row is written with 16-byte stores: even line offsets, aligned base
+    const bool rows_ok = s.om.es != 1 || (s.om.s1 % 2 == 0 && s.om.s2 % 2 == 0 &&
+                                          reinterpret_cast<std::uintptr_t>(io.dst) % 16 == 0);
+    if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && rows_ok && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = io.src_rt == RealT::f32 ? try_r2c_wave<float>(s, io, tw, st) : try_r2c_wave<__half>(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;

@@ -1,15 +1,14 @@
 // Fused correlation pass of FwdImageInvFFT.
 //
-// Contiguous lines (the 2-D second axis) with P in {256, 512, 1024}, a window
-// origin that is a multiple of 64 placed at register rotation 0 or E/2, and
-// N = P*Q <= 8192 run the workgroup-per-column kernel of xcorr_col.hpp (or,
-// with FASTFFT_XCORR_KERNEL=1, the wave-per-column kernel of xcorr_wave.hpp);
-// every other shape runs the generic line kernel.
+// Lines of a single index (the 2-D second axis) with P in {256, 512, 1024}, a
+// window origin that is a multiple of 64 placed at register rotation 0 or E/2,
+// N = P*Q <= 8192 and a contiguous, 16-byte aligned image column run the
+// workgroup-per-column kernel of xcorr_col.hpp; every other shape runs the
+// generic line kernel.
 #include <cstdint>
 
 #include "launch_common.hpp"
 #include "xcorr_col.hpp"
-#include "xcorr_wave.hpp"
 
 namespace fastfft_amd {
 
@@ -27,14 +26,6 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
     return hipGetLastError( );
 }
 
-template <int P, int Q, int QB, int RS, typename TImg>
-static hipError_t go_xcorr_wave_qb(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    k_xcorr_wave<P, Q, QB, RS, TImg><<<dim3((unsigned)s.nlines), dim3(64), 0, st>>>(
-            static_cast<const float2*>(io.src), s.im.s1, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
-            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw);
-    return hipGetLastError( );
-}
-
 // One workgroup per column; columns 0 and nlines-1 share workgroup 0 when they
 // are the DC and Nyquist columns of a real transform (FASTFFT_XCORR_PACK=0
 // gives them a workgroup each, for tests).
@@ -43,27 +34,13 @@ static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* 
     const int pack   = s.real_end_lines && s.nlines >= 2 && env_int("FASTFFT_XCORR_PACK", 1) ? 1 : 0;
     const int blocks = s.nlines - pack;
     k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks), dim3(64 * Q), 0, st>>>(
-            static_cast<const float2*>(io.src), s.im.s1, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
-            static_cast<float2*>(io.dst), s.om.s1, s.M, s.o, io.scale, tw, s.nlines - 1, pack);
+            static_cast<const float2*>(io.src), s.im.s1, s.im.es, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
+            static_cast<float2*>(io.dst), s.om.s1, s.om.es, s.M, s.o, io.scale, tw, s.nlines - 1, pack);
     return hipGetLastError( );
 }
 
-template <int P, int Q, int RS, typename TImg>
-static hipError_t go_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    // FASTFFT_XCORR_KERNEL: 0 = one workgroup of Q waves per column (default),
-    // 1 = one wave per column (P = 512, Q = 8 only; kept for comparison)
-    if constexpr ( P == 512 && Q == 8 ) {
-        if ( env_int("FASTFFT_XCORR_KERNEL", 0) == 1 ) {
-            if ( env_int("FASTFFT_XCORR_QB", 2) == 1 )
-                return go_xcorr_wave_qb<P, Q, 1, RS, TImg>(s, io, tw, st);
-            return go_xcorr_wave_qb<P, Q, 2, RS, TImg>(s, io, tw, st);
-        }
-    }
-    return go_xcorr_col<P, Q, RS, TImg>(s, io, tw, st);
-}
-
 template <int P, typename TImg>
-static hipError_t try_xcorr_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+static hipError_t try_xcorr_col(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     constexpr int E = P / 64;
     if ( s.o % 64 != 0 )
         return hipErrorNotSupported;
@@ -75,9 +52,9 @@ static hipError_t try_xcorr_wave(const Stage& s, const StageIO& io, const float2
         }
         else {
             if ( rs == 0 )
-                return go_xcorr_wave<P, Q, 0, TImg>(s, io, tw, st);
+                return go_xcorr_col<P, Q, 0, TImg>(s, io, tw, st);
             if ( rs == E / 2 )
-                return go_xcorr_wave<P, Q, E / 2, TImg>(s, io, tw, st);
+                return go_xcorr_col<P, Q, E / 2, TImg>(s, io, tw, st);
             return hipErrorNotSupported;
         }
     };
@@ -90,19 +67,19 @@ static hipError_t try_xcorr_wave(const Stage& s, const StageIO& io, const float2
     }
 }
 
-static bool contiguous_lines(const HostLineMap& m) { return m.es == 1 && m.n2 == 1 && m.s2 == 0; }
+static bool single_index_lines(const HostLineMap& m) { return m.n2 == 1 && m.s2 == 0; }
 
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    // 16-byte image loads: aligned image base and column starts
-    const int  per     = io.img_ct == CpxT::c32h ? 4 : 2;
-    const bool wave_ok = contiguous_lines(s.im) && contiguous_lines(s.gm) && contiguous_lines(s.om) &&
-                         reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 && s.gm.s1 % per == 0 &&
-                         ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
-    if ( wave_ok ) {
-        hipError_t e = hipErrorNotSupported;
+    // 16-byte image loads: contiguous columns, aligned image base and column starts
+    const int  per    = io.img_ct == CpxT::c32h ? 4 : 2;
+    const bool col_ok = single_index_lines(s.im) && single_index_lines(s.om) && single_index_lines(s.gm) &&
+                        s.gm.es == 1 && reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 && s.gm.s1 % per == 0 &&
+                        ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
+    if ( col_ok ) {
+        hipError_t e    = hipErrorNotSupported;
         auto       pick = [&](auto pc) -> hipError_t {
             constexpr int P = decltype(pc)::value;
-            return io.img_ct == CpxT::c64 ? try_xcorr_wave<P, float2>(s, io, tw, st) : try_xcorr_wave<P, __half2>(s, io, tw, st);
+            return io.img_ct == CpxT::c64 ? try_xcorr_col<P, float2>(s, io, tw, st) : try_xcorr_col<P, __half2>(s, io, tw, st);
         };
         if ( s.P == 256 )
             e = pick(std::integral_constant<int, 256>{ });

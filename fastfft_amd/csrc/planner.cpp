@@ -156,6 +156,13 @@ static long ws_ld(int n) {
     return n + pad;
 }
 
+// Leading dimension of a row-major workspace array of n complex values per row:
+// even, so that every row starts 16-byte aligned for the 16-byte row accesses.
+static long ws_ld_rows(int n) {
+    const long ld = ws_ld(n);
+    return ld + (ld & 1);
+}
+
 static HostLineMap lm(long s1, long s2, long es, int n2) {
     HostLineMap m;
     m.s1 = s1;
@@ -348,34 +355,38 @@ static PlanError xcorr_stages_2d(const PlanState& p, std::vector<Stage>& st) {
     }
     const Dims &in = p.fwd_in, &mid = p.fwd_out, &out = p.inv_out;
     const int   pitch = 2 * in.w;
-    Stage       s1    = make_stage(StageKind::fwd_real, p.fwd_change == SizeChange::increase ? "r2c_increase_XY" : "r2c_none_XY",
-                                   mid.x, in.x, 0, 0, mid.w, in.y);
-    s1.src            = Buf::user_in;
-    s1.src_elem       = Elem::real_input_type;
-    s1.im             = lm(pitch, 0, 1, 1);
-    s1.dst            = Buf::ws_a;
-    s1.dst_elem       = Elem::cpx_f32;
-    s1.om             = lm(1, 0, ws_ld(in.y), 1);
+    // The two intermediates are row-major, A[y][kx] (ldA) and B[j][kx] (ldB),
+    // so that both x-passes read and write whole rows; the fused y-pass
+    // gathers its template column from A and scatters its output column to B.
+    const long ldA = ws_ld_rows(mid.w), ldB = ws_ld_rows(mid.w);
+    Stage      s1  = make_stage(StageKind::fwd_real, p.fwd_change == SizeChange::increase ? "r2c_increase_XY" : "r2c_none_XY",
+                                mid.x, in.x, 0, 0, mid.w, in.y);
+    s1.src         = Buf::user_in;
+    s1.src_elem    = Elem::real_input_type;
+    s1.im          = lm(pitch, 0, 1, 1);
+    s1.dst         = Buf::ws_a;
+    s1.dst_elem    = Elem::cpx_f32;
+    s1.om          = lm(ldA, 0, 1, 1);
     st.push_back(s1);
     const bool dec = p.inv_change == SizeChange::decrease;
     Stage      s2  = make_stage(StageKind::xcorr,
                                 dec ? (p.fwd_change == SizeChange::increase ? "xcorr_fwd_increase_inv_decrease" : "xcorr_fwd_none_inv_decrease")
                                     : "generic_fwd_increase_op_inv_none",
                                 mid.y, in.y, out.y, axis_origin(p, 1, mid.y, out.y), 0, mid.w);
-    s2.src         = Buf::ws_a;
-    s2.src_elem    = Elem::cpx_f32;
-    s2.im          = lm(ws_ld(in.y), 0, 1, 1);
-    s2.gm          = lm(mid.y, 0, 1, 1);
-    s2.dst         = Buf::ws_b;
-    s2.dst_elem    = Elem::cpx_f32;
-    s2.om          = lm(ws_ld(out.y), 0, 1, 1);
+    s2.src            = Buf::ws_a;
+    s2.src_elem       = Elem::cpx_f32;
+    s2.im             = lm(1, 0, ldA, 1);
+    s2.gm             = lm(mid.y, 0, 1, 1);
+    s2.dst            = Buf::ws_b;
+    s2.dst_elem       = Elem::cpx_f32;
+    s2.om             = lm(1, 0, ldB, 1);
     s2.real_end_lines = mid.x % 2 == 0 && mid.w >= 2;
     st.push_back(s2);
     Stage s3    = make_stage(StageKind::inv_c2r, dec ? "c2r_decrease_XY" : "c2r_none_XY", mid.x, 0, out.x,
                              axis_origin(p, 0, mid.x, out.x), 0, out.y);
     s3.src      = Buf::ws_b;
     s3.src_elem = Elem::cpx_f32;
-    s3.im       = lm(1, 0, ws_ld(out.y), 1);
+    s3.im       = lm(ldB, 0, 1, 1);
     s3.dst      = Buf::user_out;
     s3.dst_elem = Elem::real_input_type;
     s3.om       = lm(2 * out.w, 0, 1, 1);

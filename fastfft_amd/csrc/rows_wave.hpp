@@ -2,21 +2,26 @@
 //
 //   k_r2c_rows_wave  R2C x-pass of a zero-padded real row (the reference's
 //                    block_fft_kernel_R2C_INCREASE_XY / _NONE_XY,
-//                    src/fastfft/FastFFT.cu:1055-1166): for each q the wave q of the
-//                    row's block computes X[Q k' + q] = FFT_P(x[n] W_N^{n q})[k'] and
-//                    stores the kx <= N/2 values transposed, A[kx][row].
+//                    src/fastfft/FastFFT.cu:1055-1166): wave q of the row's
+//                    workgroup computes X[Q k' + q] = FFT_P(x[n] W_N^{n q})[k'] and
+//                    the kx < nout values are stored, either as one contiguous
+//                    row (unit output stride, written through LDS with 16-byte
+//                    stores) or transposed, one value per output line.
 //   k_c2r_rows_wave  trimmed C2R x-pass (block_fft_kernel_C2R_DECREASE_XY / _NONE_XY,
 //                    :1641-1692, without the overlapping-row writes): wave q computes
-//                    U_q = IFFT_P of the Hermitian-extended X[Q s + q] gathered from
-//                    B[kx][row]; the real part of
+//                    U_q = IFFT_P of the Hermitian-extended X[Q s + q]; the real part of
 //                       y[o + j] = sum_q e^{2 pi i (o + j) q / N} U_q[(o + j) mod P]
 //                    is reduced over the waves through LDS and stored as one row.
+//                    A contiguous input row is first copied into LDS with
+//                    coalesced loads; a strided one is gathered directly.
 //
-// One block per row: Q waves (Q <= 16), each holding a P-point sub-transform
-// with E = P/64 values per lane.  Blocks are mapped to rows so that the blocks
-// one XCD runs (block b runs on XCD b mod 8) own a contiguous range of rows:
-// the transposed accesses A[kx][row] / B[kx][row] of neighbouring rows share
-// cache lines, which then meet in that XCD's L2.
+// One workgroup per row: Q waves (Q <= 16), each holding a P-point sub-transform
+// with E = P/64 values per lane.  The Stockham twiddles (W_P^k) and the per-lane
+// twiddles W_N^{t q} come from LDS copies made once per workgroup, the
+// wave-uniform W_N^{64 i q} from scalar loads.  Workgroups are mapped to rows so
+// that the workgroups one XCD runs (workgroup b runs on XCD b mod 8) own a
+// contiguous range of rows, which keeps strided accesses of neighbouring rows
+// in one L2.
 #pragma once
 
 #include "line_kernels.hpp"
@@ -34,33 +39,84 @@ __device__ __forceinline__ int xcd_row(int b, int nrows) {
     return (b % kXcds) * (nrows / kXcds) + b / kXcds;
 }
 
+// LDS slot of element k = Q k' + q of a row: the Q decimated sequences one after
+// another, each XOR-swizzled inside blocks of 64 slots, so that wave q's reads
+// or writes of consecutive k' and the row-order accesses of two consecutive k
+// per lane are free of bank conflicts.
+template <int P, int Q>
+__device__ __forceinline__ int dec_slot(int k) {
+    const int q = k % Q, kp = k / Q;
+    return q * P + (kp ^ (((q >> 1) * (128 / Q)) & 63));
+}
+
+// LDS copies of W_P^k (k < P) and W_N^k (k < 64 Q) from the multi-resolution table.
+template <int P, int Q>
+__device__ __forceinline__ void stage_row_twiddles(float2* s_twp, float2* s_twn, const float2* __restrict__ twm) {
+    constexpr int N = P * Q, NT = 64 * Q;
+    for ( int k = threadIdx.x; k < P; k += NT )
+        s_twp[k] = twm[P + k];
+    for ( int k = threadIdx.x; k < NT; k += NT )
+        s_twn[k] = twm[N + k];
+}
+
 template <int P, int Q, typename TIn>
 __global__ void __launch_bounds__(64 * Q)
         k_r2c_rows_wave(const TIn* __restrict__ in, LineMap im, int L, float2* __restrict__ out, LineMap om, int nout,
                         int nrows, float pre_scale, const float2* __restrict__ tw) {
-    constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
-    __shared__ float2 lds[Q * WaveFFT<P>::SLOTS];
-    const int q   = threadIdx.x / T;
-    const int t   = threadIdx.x % T;
-    const int row = xcd_row(blockIdx.x, nrows);
-    WaveFFT<P> fft;
-    fft.init(t, tw);
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q;
+    using FFT       = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "a row must fit the exchange buffers");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_twn[NT];
+    __shared__ float  s_row[P];
+    const int     q   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t   = threadIdx.x % T;
+    const int     row = xcd_row(blockIdx.x, nrows);
+    const float2* twm = multi_twiddles(tw);
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
     const TIn* rp = in + im.base(row);
-    float2     v[E];
+    for ( int n = threadIdx.x; n < L; n += NT )
+        s_row[n] = load_real(rp, n * im.es) * pre_scale;
+    __syncthreads( );
+    FFT fft;
+    fft.init_p(t, s_twp);
+    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    float2       v[E];
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
-        const int   n = t + T * i;
-        const float x = n < L ? load_real(rp, n * im.es) * pre_scale : 0.f;
-        const float2 w = tw[((n * q) & (N - 1)) * TWS];
-        v[i]           = make_float2(x * w.x, x * w.y);
+        const int    n = t + T * i;
+        const float  x = n < L ? s_row[n] : 0.f;
+        const float2 c = twm[N + ((T * i * q) & (N - 1))]; // W_N^{64 i q}, wave-uniform
+        v[i]           = cmul(make_float2(x * c.x, x * c.y), lw);
     }
-    fft.template run<-1>(v, lds + q * WaveFFT<P>::SLOTS, t);
+    fft.template run<-1>(v, lds + q * FFT::SLOTS, t);
     float2* op = out + om.base(row);
+    if ( om.es == 1 ) {
+        // row order through LDS, then 16-byte stores (op is 16-byte aligned)
+        __syncthreads( );
 #pragma unroll
-    for ( int i = 0; i < E; i++ ) {
-        const int k = Q * (t + T * i) + q;
-        if ( k < nout )
-            op[k * om.es] = v[i];
+        for ( int i = 0; i < E; i++ )
+            lds[dec_slot<P, Q>(Q * (t + T * i) + q)] = v[i];
+        __syncthreads( );
+        for ( int m = threadIdx.x; 2 * m < nout; m += NT ) {
+            const float2 u0 = lds[dec_slot<P, Q>(2 * m)];
+            if ( 2 * m + 1 < nout ) {
+                const float2 u1                      = lds[dec_slot<P, Q>(2 * m + 1)];
+                reinterpret_cast<float4*>(op)[m] = make_float4(u0.x, u0.y, u1.x, u1.y);
+            }
+            else {
+                op[2 * m] = u0;
+            }
+        }
+    }
+    else {
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int k = Q * (t + T * i) + q;
+            if ( k < nout )
+                op[k * om.es] = v[i];
+        }
     }
 }
 
@@ -68,49 +124,60 @@ template <int P, int Q, int RS, typename TOut>
 __global__ void __launch_bounds__(64 * Q)
         k_c2r_rows_wave(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o, int nrows,
                         float post_scale, const float2* __restrict__ tw) {
-    constexpr int T = 64, E = P / T, N = P * Q, TWS = kTwiddleN / N;
-    __shared__ float2 lds[Q * WaveFFT<P>::SLOTS];
-    const int q   = threadIdx.x / T;
-    const int t   = threadIdx.x % T;
-    const int row = xcd_row(blockIdx.x, nrows);
-    WaveFFT<P> fft;
-    fft.init(t, tw);
-    const float2* cp = in + im.base(row);
-    float2        v[E];
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
+    using FFT       = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "a row must fit the exchange buffers");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_twn[NT];
+    const int     q   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t   = threadIdx.x % T;
+    const int     row = xcd_row(blockIdx.x, nrows);
+    const float2* twm = multi_twiddles(tw);
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
+    const float2* cp     = in + im.base(row);
+    const bool    staged = im.es == 1;
+    if ( staged ) {
+        for ( int k = threadIdx.x; k <= H; k += NT )
+            lds[dec_slot<P, Q>(k)] = cp[k];
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_p(t, s_twp);
+    // X[k] for k <= N/2, conj(X[N - k]) above
+    float2 v[E];
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
-        const int k = Q * (t + T * i) + q;
-        float2    z;
-        if ( k <= N / 2 ) {
-            z = cp[k * im.es];
-        }
-        else {
-            z   = cp[(N - k) * im.es];
+        const int  k  = Q * (t + T * i) + q;
+        const bool lo = k <= H;
+        const int  kk = lo ? k : N - k;
+        float2     z  = staged ? lds[dec_slot<P, Q>(kk)] : cp[kk * im.es];
+        if ( ! lo )
             z.y = -z.y;
-        }
         v[i] = z;
     }
-    float2* wl = lds + q * WaveFFT<P>::SLOTS;
+    __syncthreads( );
+    float2* wl = lds + q * FFT::SLOTS;
     fft.template run<+1>(v, wl, t);
-    // this wave's real contribution to every output sample, into its own LDS region
-    float*    red   = reinterpret_cast<float*>(wl);
-    const int omask = ((o % N) + N) % N;
+    // this wave's real share of every output sample, into its own LDS region:
+    // Re(U_q[(o + j) mod P] conj(W_N^{(o + j) q})), W_N^{(o+t+64i) q} = W_N^{o q} W_N^{t q} W_N^{64 i q}
+    float*       red   = reinterpret_cast<float*>(wl);
+    const int    omask = ((o % N) + N) % N;
+    const float2 s     = cmul(s_twn[t * q], twm[N + ((omask * q) & (N - 1))]);
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
-        const int    j = t + T * i;
+        const float2 w = cmul(s, twm[N + ((T * i * q) & (N - 1))]);
         const float2 u = v[(i + RS) % E];
-        // e^{+2 pi i (o + j) q / N} = conj(W_N^{(o + j) q})
-        const float2 w = tw[(((omask + j) * q) & (N - 1)) * TWS];
-        red[j]         = u.x * w.x + u.y * w.y;
+        red[t + T * i] = u.x * w.x + u.y * w.y;
     }
     __syncthreads( );
     TOut* op = out + om.base(row);
-    for ( int j = threadIdx.x; j < M; j += blockDim.x ) {
-        float s = 0.f;
+    for ( int j = threadIdx.x; j < M; j += NT ) {
+        float s2 = 0.f;
 #pragma unroll
         for ( int w = 0; w < Q; w++ )
-            s += reinterpret_cast<const float*>(lds + w * WaveFFT<P>::SLOTS)[j];
-        store_real(op, j * om.es, s * post_scale);
+            s2 += reinterpret_cast<const float*>(lds + w * FFT::SLOTS)[j];
+        store_real(op, j * om.es, s2 * post_scale);
     }
 }
 

@@ -24,19 +24,9 @@
 // same amount of work.  Those two columns are stored with a zero imaginary part.
 #pragma once
 
-#include "wave_fft.hpp"
+#include "rows_wave.hpp"
 
 namespace fastfft_amd {
-
-// Slot of image element k = Q k' + q in the staging buffer: the Q decimated
-// sequences one after another, each XOR-swizzled inside blocks of 64 slots, so
-// that the coalesced staging writes (consecutive k, two or four per lane) and
-// the reads of wave q (consecutive k') are both free of LDS bank conflicts.
-template <int P, int Q>
-__device__ __forceinline__ int img_slot(int k) {
-    const int q = k % Q, kp = k / Q;
-    return q * P + (kp ^ (((q >> 1) * (128 / Q)) & 63));
-}
 
 // 16-byte loads of an image column: PER complex values per load.
 template <typename TImg>
@@ -68,8 +58,11 @@ struct ImgStage<__half2> {
 };
 
 // Grid: nlines - 1 workgroups when pack_ends (columns 0 and last_line share
-// workgroup 0), else nlines.  a, img, out: column c starts at c*lda, c*ldg,
-// c*ldo with unit element stride; img and img + ldg must be 16-byte aligned.
+// workgroup 0), else nlines.  Element n of column c: a[c*lda + n*a_es],
+// img[c*ldg + n] (img and img + ldg 16-byte aligned), out[c*ldo + n*o_es].
+// With a_es, o_es > 1 (row-major template spectrum and output, as the x-pass
+// kernels prefer) the template gather and the output scatter are the only
+// strided accesses.
 //
 // Every table a wave needs is copied into LDS once per workgroup with
 // coalesced loads (W_P^k for the Stockham passes, W_N^k for k < 64 Q for the
@@ -78,9 +71,9 @@ struct ImgStage<__half2> {
 // memory traffic is the coalesced column data.
 template <int P, int Q, int RS, typename TImg>
 __global__ void __launch_bounds__(64 * Q)
-        k_xcorr_col(const float2* __restrict__ a, long lda, int L, const TImg* __restrict__ img, long ldg,
-                    float2* __restrict__ out, long ldo, int M, int o, float intra_scale, const float2* __restrict__ tw,
-                    int last_line, int pack_ends) {
+        k_xcorr_col(const float2* __restrict__ a, long lda, long a_es, int L, const TImg* __restrict__ img, long ldg,
+                    float2* __restrict__ out, long ldo, long o_es, int M, int o, float intra_scale,
+                    const float2* __restrict__ tw, int last_line, int pack_ends) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, NW = T * Q;
     constexpr int PER = ImgStage<TImg>::PER;
     using FFT         = WaveFFT<P>;
@@ -93,8 +86,10 @@ __global__ void __launch_bounds__(64 * Q)
     __shared__ float2 s_tpl[P];  // template column, first L values
     const int      q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int      t    = threadIdx.x % T;
-    const long     col  = blockIdx.x;
-    const bool     ends = pack_ends && blockIdx.x == 0;
+    // each XCD takes a contiguous range of columns, so that the strided template
+    // gather and output scatter of neighbouring columns share lines in one L2
+    const long     col  = xcd_row(blockIdx.x, gridDim.x);
+    const bool     ends = pack_ends && col == 0;
     const float2*  twm  = multi_twiddles(tw);
 
     // ---- stage: twiddle tables, template column, image column
@@ -106,7 +101,7 @@ __global__ void __launch_bounds__(64 * Q)
         const float2* ac = a + col * lda;
         const float2* a1 = a + (long)last_line * lda;
         for ( int n = threadIdx.x; n < L; n += NT )
-            s_tpl[n] = ends ? make_float2(ac[n].x, a1[n].x) : ac[n];
+            s_tpl[n] = ends ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
     }
     if ( ! ends ) {
         const TImg* gc = img + col * ldg;
@@ -117,7 +112,7 @@ __global__ void __launch_bounds__(64 * Q)
             ImgStage<TImg>::load(gc, m, f);
 #pragma unroll
             for ( int e = 0; e < PER; e++ )
-                lds[img_slot<P, Q>(PER * m + e)] = f[e];
+                lds[dec_slot<P, Q>(PER * m + e)] = f[e];
         }
     }
     __syncthreads( );
@@ -136,7 +131,7 @@ __global__ void __launch_bounds__(64 * Q)
     if ( ! ends ) {
 #pragma unroll
         for ( int i = 0; i < E; i++ )
-            g[i] = lds[img_slot<P, Q>(Q * (t + T * i) + q)];
+            g[i] = lds[dec_slot<P, Q>(Q * (t + T * i) + q)];
     }
     __syncthreads( );
     float2* wl = lds + q * FFT::SLOTS;
@@ -197,11 +192,11 @@ __global__ void __launch_bounds__(64 * Q)
         for ( int w = 0; w < Q; w++ )
             acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
         if ( ! ends ) {
-            out[col * ldo + j] = acc;
+            out[col * ldo + j * o_es] = acc;
         }
         else {
-            out[j]                         = make_float2(acc.x, 0.f);
-            out[(long)last_line * ldo + j] = make_float2(acc.y, 0.f);
+            out[j * o_es]                         = make_float2(acc.x, 0.f);
+            out[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
         }
     }
 }

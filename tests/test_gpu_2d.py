@@ -268,24 +268,21 @@ def test_xcorr_random(tin, n, nout, origin):
     assert O.rel_l2(got, want) < TOL
 
 
-# the fused column kernels (P in {256, 512, 1024}, origin a multiple of 64) against the oracle: the
-# workgroup-per-column kernel with the DC and Nyquist columns packed into one workgroup (default) and
-# unpacked, the wave-per-column kernel (P = 512, Q = 8 shapes) at both register batch sizes, and the
-# generic kernel forced, so every path stays pinned; the image spectra are random, not Hermitian
+# the fused column kernel (P in {256, 512, 1024}, origin a multiple of 64) against the oracle, with
+# the DC and Nyquist columns packed into one workgroup (default) and unpacked, and the generic kernel
+# forced, so every path stays pinned; the image spectra are random, not Hermitian
 WAVE_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((48, 384), (64, 4096), (32, 512), (0, 0)),
               ((40, 200), (64, 2048), (24, 256), (-12, -128)), ((32, 300), (32, 8192), (32, 512), (0, -256)),
               ((24, 1000), (32, 4096), (16, 1024), (0, -512)), ((16, 130), (32, 1024), (16, 256), (0, 128))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_pack", "wave_qb1", "wave_qb2", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_pack", "generic"])
 @pytest.mark.parametrize("tin,n,nout,origin", WAVE_CASES)
-def test_xcorr_wave_kernel(tin, n, nout, origin, mode, monkeypatch):
-    wave = {"FASTFFT_XCORR_KERNEL": "1"}
-    env = {"default": {}, "no_pack": {"FASTFFT_XCORR_PACK": "0"}, "wave_qb1": {**wave, "FASTFFT_XCORR_QB": "1"},
-           "wave_qb2": {**wave, "FASTFFT_XCORR_QB": "2"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
+def test_xcorr_column_kernel(tin, n, nout, origin, mode, monkeypatch):
+    env = {"default": {}, "no_pack": {"FASTFFT_XCORR_PACK": "0"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
     for k, v in env.items():
         monkeypatch.setenv(k, v)
-    H.log(f"xcorr wave {tin} in {n} -> {nout} at {origin} mode={mode}")
+    H.log(f"xcorr column {tin} in {n} -> {nout} at {origin} mode={mode}")
     rng = np.random.default_rng(tin[1] * 7 + n[1])
     t = rng.standard_normal((tin[1], tin[0])).astype(np.float32)
     w = n[0] // 2 + 1

@@ -193,6 +193,7 @@ def test_bounds_3d(fin, fout, iin, iout, origin):
 
 def test_workspace_is_reported_without_gpu():
     ft = plan((384, 384, 1), (4096, 4096, 1), (4096, 4096, 1), (512, 512, 1))
-    # ws_a: A[2049][384], ws_b: B[2049][512] complex floats
-    assert ft.WorkspaceBytes() == 8 * (2049 * 384 + 2049 * 512)
+    # correlation intermediates, row-major with an even leading dimension of 2050 complex floats:
+    # ws_a: A[384 rows][2049 kx], ws_b: B[512 rows][2049 kx] (the last row ends at kx = 2048)
+    assert ft.WorkspaceBytes() == 8 * ((383 * 2050 + 2049) + (511 * 2050 + 2049))
     assert ft.GetDeviceBufferPointer() == 0  # allocated at the first transform only
