@@ -86,6 +86,7 @@ def lib():
         "fastfft_fwd": [P, V, V, _Op, _Op],
         "fastfft_inv": [P, V, V, _Op, _Op],
         "fastfft_fwd_image_inv": [P, V, V, V, _Op, _Op, _Op],
+        "fastfft_fwd_image_inv_argmax": [P, V, V, V, _Op, _Op, _Op, V, V],
         "fastfft_wait": [P],
         "fastfft_copy_device_to_host_and_synchronize": [P, V, I],
         "fastfft_get_memory_sizes": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
@@ -226,6 +227,18 @@ class FourierTransformer:
         self._bind_torch_stream()
         _check(lib().fastfft_fwd_image_inv(self._h, _ptr(input_ptr), _ptr(image_to_search), _ptr(output_ptr),
                                            pre_op or Noop(), intra_op or ConjMul(), post_op or Noop()))
+
+    def FwdImageInvFFTArgMax(self, input_ptr, image_to_search, output_ptr, peak_xy, value, pre_op=None, intra_op=None,
+                             post_op=None):
+        """Extension: FwdImageInvFFT plus the argmax of the output window, asynchronously.
+
+        peak_xy: device int32[2] receiving (x, y) of the first maximum in row-major order;
+        value: device float32[1] receiving its value.  2-D plans only.
+        """
+        self._bind_torch_stream()
+        _check(lib().fastfft_fwd_image_inv_argmax(self._h, _ptr(input_ptr), _ptr(image_to_search), _ptr(output_ptr),
+                                                  pre_op or Noop(), intra_op or ConjMul(), post_op or Noop(),
+                                                  _ptr(peak_xy), _ptr(value)))
 
     def Wait(self) -> None:
         _check(lib().fastfft_wait(self._h))

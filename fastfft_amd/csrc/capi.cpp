@@ -71,7 +71,7 @@ struct fastfft_plan_s {
     void*       ws     = nullptr; // workspace: ws_a then ws_b (float2)
     size_t      ws_a = 0, ws_b = 0;
     int         ws_device = -1;
-    float*      argmax_scratch = nullptr; // (1 + 512) floats then (2 + 512) ints
+    void*       argmax_scratch = nullptr; // argmax row keys, partials and result slots
     // CopyDeviceToHostAndSynchronize bookkeeping (FastFFT.cu:947-1009)
     void* last_buffer     = nullptr;
     int   stage_completed = 0; // 0: none, 3: forward, 7: inverse / correlation
@@ -184,7 +184,8 @@ static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stag
 }
 
 static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, const void* image, float pre, float intra,
-                                 float post, int first = 0, int last = -1) {
+                                 float post, int first = 0, int last = -1, const ArgmaxSlots* amax = nullptr,
+                                 bool* amax_fused = nullptr) {
     const std::vector<Stage>* sp = nullptr;
     if ( auto rc = get_stages(p, opi, &sp) )
         return rc;
@@ -237,7 +238,11 @@ static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, c
             case StageKind::inv_cpx:
             case StageKind::inv_c2r:
                 io.scale = (i + 1 == stages.size( )) ? post : 1.f;
-                e        = launch_inv(s, io, tw, p->stream);
+                if ( s.kind == StageKind::inv_c2r && i + 1 == stages.size( ) ) {
+                    io.amax       = amax;
+                    io.amax_fused = amax_fused;
+                }
+                e = launch_inv(s, io, tw, p->stream);
                 break;
             case StageKind::xcorr:
                 io.scale = intra;
@@ -248,6 +253,36 @@ static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, c
         if ( e != hipSuccess )
             return hip_fail(e, ("launch of stage " + s.ref_kernel).c_str( ));
     }
+    return ok( );
+}
+
+// Device scratch of the argmax epilogues:
+//   kArgmaxPartials 64-bit row keys of the argmax computed by the C2R pass,
+//   then the partial values and indices of the two-kernel epilogue and its result slots.
+struct ArgmaxScratch {
+    ArgmaxSlots fused;
+    float*      pv  = nullptr;
+    int*        pi  = nullptr;
+    float*      rv  = nullptr;
+    int*        rxy = nullptr;
+};
+
+static fastfft_status argmax_scratch(fastfft_plan plan, ArgmaxScratch& a) {
+    const size_t keys  = sizeof(unsigned long long) * kArgmaxPartials;
+    const size_t bytes = keys + sizeof(float) * (kArgmaxPartials + 1) + sizeof(int) * (kArgmaxPartials + 2);
+    if ( ! plan->argmax_scratch ) {
+        hipError_t e = hipMalloc(&plan->argmax_scratch, bytes);
+        if ( e != hipSuccess ) {
+            plan->argmax_scratch = nullptr;
+            return hip_fail(e, "hipMalloc(argmax scratch)");
+        }
+    }
+    char* base   = static_cast<char*>(plan->argmax_scratch);
+    a.fused.keys = reinterpret_cast<unsigned long long*>(base);
+    a.pv         = reinterpret_cast<float*>(base + keys);
+    a.rv         = a.pv + kArgmaxPartials;
+    a.pi         = reinterpret_cast<int*>(a.rv + 1);
+    a.rxy        = a.pi + kArgmaxPartials;
     return ok( );
 }
 
@@ -409,6 +444,45 @@ fastfft_status fastfft_fwd_image_inv(fastfft_plan plan, void* in, const void* im
     return ok( );
 }
 
+fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const void* image_to_search, void* out,
+                                            fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op, int* peak_xy,
+                                            float* value) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! in || ! image_to_search || ! peak_xy || ! value )
+        return fail(FASTFFT_INVALID_ARGUMENT, "null input, image_to_search, peak_xy or value pointer");
+    if ( auto rc = op_kind_check(pre_op, true, false, "the FwdImageInvFFT pre op") )
+        return rc;
+    if ( intra_op.kind != FASTFFT_OP_CONJ_MUL && intra_op.kind != FASTFFT_OP_CONJ_MUL_THEN_SCALE )
+        return fail(FASTFFT_INVALID_ARGUMENT, "FwdImageInvFFT needs an intra op (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
+    if ( auto rc = op_kind_check(post_op, true, false, "the FwdImageInvFFT post op") )
+        return rc;
+    if ( auto rc = ensure_validated(plan) )
+        return rc;
+    if ( plan->st.transform_dimension != 2 )
+        return fail(FASTFFT_UNSUPPORTED, "FwdImageInvFFT with argmax is implemented for 2D plans");
+    ArgmaxScratch a;
+    if ( auto rc = argmax_scratch(plan, a) )
+        return rc;
+    a.fused.value = value;
+    a.fused.xy    = peak_xy;
+    void* dst     = out ? out : in;
+    bool  fused   = false;
+    if ( auto rc = run_stages(plan, 2, in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), 0, -1,
+                              &a.fused, &fused) )
+        return rc;
+    plan->last_buffer     = dst;
+    plan->stage_completed = 7;
+    if ( ! fused ) {
+        const Dims& o = plan->st.inv_out;
+        hipError_t  e = launch_argmax(dst, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, o.x, o.y,
+                                      2 * o.w, a.pv, a.pi, value, peak_xy, plan->stream);
+        if ( e != hipSuccess )
+            return hip_fail(e, "argmax");
+    }
+    return ok( );
+}
+
 fastfft_status fastfft_wait(fastfft_plan plan) {
     if ( auto rc = check_plan(plan) )
         return rc;
@@ -486,33 +560,18 @@ fastfft_status fastfft_workspace_bytes(fastfft_plan plan, size_t* bytes) {
     return ok( );
 }
 
-// Scratch of the argmax epilogue: partial values and indices, then the result slots.
-static fastfft_status argmax_scratch(fastfft_plan plan, float** pv, int** pi, float** rv, int** rxy) {
-    if ( ! plan->argmax_scratch ) {
-        hipError_t e = hipMalloc(&plan->argmax_scratch, sizeof(float) * (kArgmaxPartials + 1) + sizeof(int) * (kArgmaxPartials + 2));
-        if ( e != hipSuccess ) {
-            plan->argmax_scratch = nullptr;
-            return hip_fail(e, "hipMalloc(argmax scratch)");
-        }
-    }
-    *pv  = plan->argmax_scratch;
-    *rv  = *pv + kArgmaxPartials;
-    *pi  = reinterpret_cast<int*>(*rv + 1);
-    *rxy = *pi + kArgmaxPartials;
-    return ok( );
-}
-
 fastfft_status fastfft_argmax(fastfft_plan plan, const void* img, int nx, int ny, int pitch, int peak[2], float* value) {
     if ( auto rc = check_plan(plan) )
         return rc;
     if ( ! img || ! peak || nx <= 0 || ny <= 0 || pitch < nx )
         return fail(FASTFFT_INVALID_ARGUMENT, "argmax: null pointer or bad extent");
-    float *pv, *rv;
-    int *  pi, *rxy;
-    if ( auto rc = argmax_scratch(plan, &pv, &pi, &rv, &rxy) )
+    ArgmaxScratch a;
+    if ( auto rc = argmax_scratch(plan, a) )
         return rc;
-    hipError_t e = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, pv, pi,
-                                 rv, rxy, plan->stream);
+    float*     rv  = a.rv;
+    int*       rxy = a.rxy;
+    hipError_t e   = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch,
+                                   a.pv, a.pi, rv, rxy, plan->stream);
     if ( e != hipSuccess )
         return hip_fail(e, "argmax launch");
     float hv = 0.f;
@@ -536,12 +595,11 @@ fastfft_status fastfft_argmax_device(fastfft_plan plan, const void* img, int nx,
         return rc;
     if ( ! img || ! peak_xy || ! value || nx <= 0 || ny <= 0 || pitch < nx )
         return fail(FASTFFT_INVALID_ARGUMENT, "argmax: null pointer or bad extent");
-    float *pv, *rv;
-    int *  pi, *rxy;
-    if ( auto rc = argmax_scratch(plan, &pv, &pi, &rv, &rxy) )
+    ArgmaxScratch a;
+    if ( auto rc = argmax_scratch(plan, a) )
         return rc;
-    hipError_t e = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, pv, pi,
-                                 value, peak_xy, plan->stream);
+    hipError_t e = launch_argmax(img, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, nx, ny, pitch, a.pv,
+                                 a.pi, value, peak_xy, plan->stream);
     return e == hipSuccess ? ok( ) : hip_fail(e, "argmax");
 }
 

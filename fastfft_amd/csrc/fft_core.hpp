@@ -62,6 +62,24 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int idx
 // contiguous (coalesced loads).
 __device__ __forceinline__ const float2* multi_twiddles(const float2* tw) { return tw + kTwiddleN; }
 
+// ---------------------------------------------------------------- argmax keys
+// 64-bit key ordered like (value, -index): the high word is the IEEE value with
+// its order made unsigned (sign bit flipped for non-negatives, all bits for
+// negatives), the low word 0xffffffff - index, so that the largest key is the
+// first maximum in index order.
+__device__ __forceinline__ unsigned long long argmax_key(float v, unsigned idx) {
+    const unsigned u  = __float_as_uint(v);
+    const unsigned hi = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((unsigned long long)hi << 32) | (0xffffffffu - idx);
+}
+
+__device__ __forceinline__ float argmax_key_value(unsigned long long k) {
+    const unsigned hi = (unsigned)(k >> 32);
+    return __uint_as_float((hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi);
+}
+
+__device__ __forceinline__ unsigned argmax_key_index(unsigned long long k) { return 0xffffffffu - (unsigned)k; }
+
 // ---------------------------------------------------------------- type conversion
 template <typename T>
 __device__ __forceinline__ float load_real(const T* p, long i);
@@ -69,6 +87,14 @@ template <>
 __device__ __forceinline__ float load_real<float>(const float* p, long i) { return p[i]; }
 template <>
 __device__ __forceinline__ float load_real<__half>(const __half* p, long i) { return __half2float(p[i]); }
+
+// v rounded to T and back, as a store of v to T and a load would give
+template <typename T>
+__device__ __forceinline__ float convert_real(float v);
+template <>
+__device__ __forceinline__ float convert_real<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float convert_real<__half>(float v) { return __half2float(__float2half_rn(v)); }
 
 template <typename T>
 __device__ __forceinline__ void store_real(T* p, long i, float v);

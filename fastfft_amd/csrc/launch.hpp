@@ -11,6 +11,15 @@ namespace fastfft_amd {
 enum class RealT { f32 = 0, f16 = 1 };
 enum class CpxT { c64 = 0, c32h = 1 };
 
+// Device slots of the argmax that a trimmed C2R x-pass can compute while it
+// stores its rows: one key per row (kArgmaxPartials at least), reduced after
+// the pass into value and xy.
+struct ArgmaxSlots {
+    unsigned long long* keys  = nullptr;
+    float*              value = nullptr;
+    int*                xy    = nullptr;
+};
+
 struct StageIO {
     const void* src = nullptr;
     void*       dst = nullptr;
@@ -19,6 +28,10 @@ struct StageIO {
     RealT       src_rt = RealT::f32, dst_rt = RealT::f32;
     CpxT        src_ct = CpxT::c64, dst_ct = CpxT::c64, img_ct = CpxT::c64;
     float       scale = 1.f; // pre (fwd), post (inv), intra (xcorr)
+    // inv_c2r only: compute the argmax of the output rows into *amax when the
+    // kernel supports it, and set *amax_fused (left unchanged otherwise)
+    const ArgmaxSlots* amax       = nullptr;
+    bool*              amax_fused = nullptr;
 };
 
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
@@ -38,6 +51,10 @@ hipError_t launch_twiddle_init(float2* tw, hipStream_t stream);
 constexpr int kArgmaxPartials = 512;
 hipError_t    launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* partial_v, int* partial_i,
                             float* result_value, int* result_xy, hipStream_t stream);
+
+// Reduces n argmax keys (fft_core.hpp) to the value and (x, y) = (index mod nx, index / nx).
+hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, float* result_value, int* result_xy,
+                              hipStream_t stream);
 
 // LDS bytes per workgroup of a line stage with sub-transform size P.
 size_t stage_lds_bytes(int P);

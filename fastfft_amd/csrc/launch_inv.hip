@@ -21,10 +21,18 @@ static hipError_t go_inv(const Stage& s, const StageIO& io, const float2* tw, hi
 // rotation 0 or E/2: one block per row, one wave per sub-transform.
 template <int P, int Q, int RS, typename TOut>
 static hipError_t go_c2r_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    // the argmax keys need one slot per row
+    const bool keys = io.amax && s.nlines <= kArgmaxPartials;
     k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
             static_cast<const float2*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
-            s.nlines, io.scale, tw);
-    return hipGetLastError( );
+            s.nlines, io.scale, tw, keys ? io.amax->keys : nullptr);
+    hipError_t e = hipGetLastError( );
+    if ( e == hipSuccess && keys ) {
+        e = launch_argmax_keys(io.amax->keys, s.nlines, s.M, io.amax->value, io.amax->xy, st);
+        if ( e == hipSuccess && io.amax_fused )
+            *io.amax_fused = true;
+    }
+    return e;
 }
 
 template <typename TOut>

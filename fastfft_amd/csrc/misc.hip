@@ -102,6 +102,40 @@ __global__ void __launch_bounds__(kArgmaxThreads)
     }
 }
 
+// First maximum of n keys (value order, then smaller index).
+__global__ void __launch_bounds__(kArgmaxThreads)
+        k_argmax_keys(const unsigned long long* __restrict__ keys, int n, int nx, float* __restrict__ rv, int* __restrict__ rxy) {
+    __shared__ unsigned long long s_best;
+    unsigned long long            best = 0;
+    for ( int i = threadIdx.x; i < n; i += kArgmaxThreads ) {
+        const unsigned long long k = keys[i];
+        best                       = k > best ? k : best;
+    }
+#pragma unroll
+    for ( int off = 32; off > 0; off >>= 1 ) {
+        const unsigned long long other = __shfl_xor(best, off);
+        best                           = other > best ? other : best;
+    }
+    if ( threadIdx.x == 0 )
+        s_best = 0;
+    __syncthreads( );
+    if ( (threadIdx.x & 63) == 0 )
+        atomicMax(&s_best, best);
+    __syncthreads( );
+    if ( threadIdx.x == 0 ) {
+        const unsigned idx = argmax_key_index(s_best);
+        rv[0]              = argmax_key_value(s_best);
+        rxy[0]             = (int)(idx % (unsigned)nx);
+        rxy[1]             = (int)(idx / (unsigned)nx);
+    }
+}
+
+hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, float* result_value, int* result_xy,
+                              hipStream_t stream) {
+    k_argmax_keys<<<1, kArgmaxThreads, 0, stream>>>(keys, n, nx, result_value, result_xy);
+    return hipGetLastError( );
+}
+
 // partial_v / partial_i: kArgmaxBlocks entries of device scratch; the result goes to
 // result_value[0] and result_xy[0..1] (device memory, may be caller buffers).
 hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* partial_v, int* partial_i,

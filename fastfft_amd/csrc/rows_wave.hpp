@@ -120,10 +120,30 @@ __global__ void __launch_bounds__(64 * Q)
     }
 }
 
+// Largest key over the workgroup, returned to thread 0 (wave reduction, then LDS).
+__device__ __forceinline__ unsigned long long block_max_key(unsigned long long key) {
+    __shared__ unsigned long long s_key;
+#pragma unroll
+    for ( int off = 32; off > 0; off >>= 1 ) {
+        const unsigned long long other = __shfl_xor(key, off);
+        key                            = other > key ? other : key;
+    }
+    if ( threadIdx.x == 0 )
+        s_key = 0;
+    __syncthreads( );
+    if ( (threadIdx.x & 63) == 0 )
+        atomicMax(&s_key, key);
+    __syncthreads( );
+    return s_key;
+}
+
+// row_keys == nullptr: no argmax; else workgroup b stores the largest argmax key
+// (fft_core.hpp) of its row, index row * M + j, to row_keys[b] for a one-workgroup
+// reduction after the launch.
 template <int P, int Q, int RS, typename TOut>
 __global__ void __launch_bounds__(64 * Q)
         k_c2r_rows_wave(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o, int nrows,
-                        float post_scale, const float2* __restrict__ tw) {
+                        float post_scale, const float2* __restrict__ tw, unsigned long long* __restrict__ row_keys) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
     using FFT       = WaveFFT<P>;
     static_assert(Q * FFT::SLOTS >= N, "a row must fit the exchange buffers");
@@ -171,13 +191,26 @@ __global__ void __launch_bounds__(64 * Q)
         red[t + T * i] = u.x * w.x + u.y * w.y;
     }
     __syncthreads( );
-    TOut* op = out + om.base(row);
+    TOut*              op   = out + om.base(row);
+    unsigned long long best = 0;
     for ( int j = threadIdx.x; j < M; j += NT ) {
         float s2 = 0.f;
 #pragma unroll
         for ( int w = 0; w < Q; w++ )
             s2 += reinterpret_cast<const float*>(lds + w * FFT::SLOTS)[j];
-        store_real(op, j * om.es, s2 * post_scale);
+        s2 *= post_scale;
+        store_real(op, j * om.es, s2);
+        if ( row_keys ) {
+            // the value as stored (rounded when the output is half precision)
+            const float stored = convert_real<TOut>(s2);
+            const auto  k      = argmax_key(stored, (unsigned)(row * M + j));
+            best               = k > best ? k : best;
+        }
+    }
+    if ( row_keys ) {
+        best = block_max_key(best);
+        if ( threadIdx.x == 0 )
+            row_keys[blockIdx.x] = best;
     }
 }
 

@@ -214,6 +214,18 @@ class FourierTransformer {
                       "FwdImageInvFFT");
     }
 
+    // Extension: FwdImageInvFFT plus the argmax of the output window into device
+    // int[2] peak_xy (x, y) and device float value, asynchronously (2-D plans).
+    template <class PreOpType = std::nullptr_t, class IntraOpType = std::nullptr_t, class PostOpType = std::nullptr_t>
+    void FwdImageInvFFTArgMax(InputType* input_ptr, OtherImageType* image_to_search, InputType* output_ptr, int* peak_xy,
+                              float* value, PreOpType pre_op = nullptr, IntraOpType intra_op = nullptr,
+                              PostOpType post_op = nullptr) {
+        static_assert(HasIntraOpFunctor<IntraOpType>, "FwdImageInvFFT needs an intra op functor (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
+        detail::check(fastfft_fwd_image_inv_argmax(plan_, input_ptr, image_to_search, output_ptr, detail::to_op(pre_op),
+                                                   detail::to_op(intra_op), detail::to_op(post_op), peak_xy, value),
+                      "FwdImageInvFFTArgMax");
+    }
+
     // Extension: argmax of a real [ny][pitch] device image, (x, y) and value.
     void ArgMax(const InputType* img, int nx, int ny, int pitch, int& peak_x, int& peak_y, float& value) {
         int pk[2];

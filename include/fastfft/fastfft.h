@@ -107,6 +107,18 @@ fastfft_status fastfft_inv(fastfft_plan plan, void* in, void* out, fastfft_op in
 fastfft_status fastfft_fwd_image_inv(fastfft_plan plan, void* in, const void* image_to_search, void* out, fastfft_op pre_op,
                                      fastfft_op intra_op, fastfft_op post_op);
 
+/* Extension: FwdImageInvFFT followed by the argmax of the (trimmed) output:
+ * the C2R x-pass reduces each row to a key as it stores it and one small
+ * kernel reduces the row keys (a separate two-kernel epilogue when that C2R
+ * kernel does not apply).  peak_xy (device int[2]) receives (x, y)
+ * of the first maximum in row-major order of the out_x by out_y window, value
+ * (device float) its value.  Asynchronous on the plan's stream and free of
+ * allocations and memsets after the first call on a plan, so it can be
+ * captured in a HIP graph.  2-D plans only. */
+fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const void* image_to_search, void* out,
+                                            fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op, int* peak_xy,
+                                            float* value);
+
 /* Wait() (FastFFT.h:124-126): synchronise the plan's stream. */
 fastfft_status fastfft_wait(fastfft_plan plan);
 

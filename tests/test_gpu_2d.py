@@ -361,3 +361,98 @@ def test_xcorr_4096_384_trim_512_centred():
     # the library's own argmax epilogue agrees
     px, py, pv = ft.ArgMax(out, 512, 512, H.F.padded_pitch(512))
     assert (px, py) == (gx, gy)
+    # the fused argmax of the trimmed C2R pass gives the same output and the same peak
+    torch, d = H.torch_dev()
+    peak = torch.zeros(2, dtype=torch.int32, device=d)
+    val = torch.zeros(1, dtype=torch.float32, device=d)
+    out2 = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.FwdImageInvFFTArgMax(a, mbuf, out2, peak, val)
+    assert np.array_equal(H.to_host(out2), H.to_host(out))
+    assert tuple(H.to_host(peak)) == (gx, gy) and H.to_host(val)[0] == np.float32(pv)
+
+
+# ------------------------------------------------------------------ fused argmax of FwdImageInvFFTArgMax
+ARGMAX_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((40, 200), (64, 2048), (24, 256), (-12, -128)),
+                ((100, 60), (256, 128), (100, 60), (-50, -30)), ((24, 1000), (32, 4096), (16, 1024), (0, -512))]
+
+
+def _argmax_case(tin, n, nout, origin, input_type="f32", seed=0):
+    torch, d = H.torch_dev()
+    rng = np.random.default_rng(seed + tin[0] * 3 + n[1])
+    t = rng.standard_normal((tin[1], tin[0])).astype(np.float32)
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((w, n[1])) + 1j * rng.standard_normal((w, n[1]))).astype(np.complex64)
+    ft = H.make_plan((*tin, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0), input_type=input_type)
+    dt = "f16" if input_type == "f16" else "f32"
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()), dt)
+    tp = H.padded_real(t, tin[0]).reshape(-1)
+    a[: tp.size] = H.to_dev(tp.astype(np.float16 if dt == "f16" else np.float32))
+    img = H.to_dev(H.complex_to_f32(spec))
+    return torch, d, ft, a, img, dt
+
+
+@pytest.mark.parametrize("mode", ["fused", "separate"])
+@pytest.mark.parametrize("tin,n,nout,origin", ARGMAX_CASES)
+def test_fused_argmax(tin, n, nout, origin, mode, monkeypatch):
+    if mode == "separate":
+        monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    H.log(f"fused argmax {tin} in {n} -> {nout} at {origin} mode={mode}")
+    torch, d, ft, a, img, dt = _argmax_case(tin, n, nout, origin)
+    pitch = H.F.padded_pitch(nout[0])
+    ref_out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), dt)
+    ft.FwdImageInvFFT(a, img, ref_out, intra_op=H.F.ConjMulThenScale(0.5))
+    rx, ry, rv = ft.ArgMax(ref_out, nout[0], nout[1], pitch)
+    host = H.real_window(H.to_host(ref_out), nout[0], nout[1])
+    assert (rx, ry) == tuple(int(v) for v in np.unravel_index(np.argmax(host), host.shape)[::-1])
+    for rep in range(3):  # the epilogue's key and counter return to zero after every call
+        out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), dt)
+        peak = torch.full((2,), -1, dtype=torch.int32, device=d)
+        val = torch.zeros(1, dtype=torch.float32, device=d)
+        ft.FwdImageInvFFTArgMax(a, img, out, peak, val, intra_op=H.F.ConjMulThenScale(0.5))
+        assert np.array_equal(H.to_host(out), H.to_host(ref_out)), rep
+        assert tuple(H.to_host(peak)) == (rx, ry), rep
+        assert H.to_host(val)[0] == np.float32(rv), rep
+
+
+def test_fused_argmax_half_output():
+    tin, n, nout, origin = (48, 384), (64, 4096), (32, 512), (0, -256)
+    H.log(f"fused argmax half output {tin} in {n} -> {nout}")
+    torch, d, ft, a, img, dt = _argmax_case(tin, n, nout, origin, input_type="f16")
+    pitch = H.F.padded_pitch(nout[0])
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), dt)
+    peak = torch.zeros(2, dtype=torch.int32, device=d)
+    val = torch.zeros(1, dtype=torch.float32, device=d)
+    ft.FwdImageInvFFTArgMax(a, img, out, peak, val)
+    rx, ry, rv = ft.ArgMax(out, nout[0], nout[1], pitch)
+    assert tuple(H.to_host(peak)) == (rx, ry) and H.to_host(val)[0] == np.float32(rv)
+
+
+def test_fused_argmax_graph_replays():
+    tin, n, nout, origin = (48, 384), (64, 4096), (32, 512), (0, -256)
+    H.log("fused argmax in a HIP graph, replayed with eager work in between")
+    torch, d, ft, a, img, dt = _argmax_case(tin, n, nout, origin)
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), dt)
+    peak = torch.zeros(2, dtype=torch.int32, device=d)
+    val = torch.zeros(1, dtype=torch.float32, device=d)
+    side = torch.cuda.Stream()
+    ft.SetStream(side.cuda_stream)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        ft.FwdImageInvFFTArgMax(a, img, out, peak, val)  # warm-up: allocations happen outside the capture
+    torch.cuda.synchronize()
+    want_out, want_peak, want_val = H.to_host(out).copy(), H.to_host(peak).copy(), H.to_host(val).copy()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        ft.FwdImageInvFFTArgMax(a, img, out, peak, val)
+    for rep in range(4):
+        out.zero_()
+        peak.fill_(-1)
+        val.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(H.to_host(out), want_out), rep
+        assert np.array_equal(H.to_host(peak), want_peak) and np.array_equal(H.to_host(val), want_val), rep
+        ft.FwdImageInvFFTArgMax(a, img, out, peak, val)  # eager call on the same plan between replays
+        torch.cuda.synchronize()
+        assert np.array_equal(H.to_host(peak), want_peak), rep
