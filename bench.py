@@ -164,6 +164,14 @@ def main() -> int:
         torch.cuda.synchronize()
         del ca, cb
 
+    # measured HBM bytes per launch of the fused y-pass (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE, calibrated
+    # against a device copy), recorded with the command that measured them
+    traffic, traffic_src = None, None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "xcorr_col_4096x384_hbm_traffic.md")
+    if (N, T, M) == (4096, 384, 512) and os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_dispatch k_xcorr_col 4096x384 trim512:"):
+                traffic, traffic_src = int(line.split(":")[1]), "profiles/xcorr_col_4096x384_hbm_traffic.md"
     # algorithmic bytes of the fused y-pass per launch: A[2049][384] in, spectrum C[2049][4096] in,
     # B[2049][512] out (complex float)
     w = N // 2 + 1
@@ -193,12 +201,13 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_xcorr<512> (fused y-pass)",
+            "kernel": "k_xcorr_col<512,8,4,float2> (fused y-pass)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": fused_bytes,
             "avg_launch_ms": round(fused_ms, 5),
         },
