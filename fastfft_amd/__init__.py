@@ -87,6 +87,8 @@ def lib():
         "fastfft_inv": [P, V, V, _Op, _Op],
         "fastfft_fwd_image_inv": [P, V, V, V, _Op, _Op, _Op],
         "fastfft_fwd_image_inv_argmax": [P, V, V, V, _Op, _Op, _Op, V, V],
+        "fastfft_set_batch": [P, ctypes.c_int, V, V, V, V, V],
+        "fastfft_fwd_image_inv_batch": [P, V, _Op, _Op, _Op],
         "fastfft_wait": [P],
         "fastfft_copy_device_to_host_and_synchronize": [P, V, I],
         "fastfft_get_memory_sizes": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
@@ -239,6 +241,25 @@ class FourierTransformer:
         _check(lib().fastfft_fwd_image_inv_argmax(self._h, _ptr(input_ptr), _ptr(image_to_search), _ptr(output_ptr),
                                                   pre_op or Noop(), intra_op or ConjMul(), post_op or Noop(),
                                                   _ptr(peak_xy), _ptr(value)))
+
+    def SetBatch(self, templates, sizes, outputs, peaks_xy=None, values=None) -> None:
+        """Extension: register templates (device buffers or pointers), their (x, y) sizes and output buffers
+        for FwdImageInvFFTBatch; peaks_xy (device int32[2n]) and values (device float32[n]) are optional."""
+        n = len(templates)
+        if len(sizes) != n or len(outputs) != n:
+            raise ValueError("templates, sizes and outputs must have the same length")
+        arr_t = (ctypes.c_void_p * max(n, 1))(*[_ptr(t) for t in templates])
+        arr_o = (ctypes.c_void_p * max(n, 1))(*[_ptr(o) for o in outputs])
+        arr_s = (ctypes.c_int * max(2 * n, 1))(*[int(v) for s in sizes for v in s])
+        self._batch_keepalive = (list(templates), list(outputs), peaks_xy, values)
+        _check(lib().fastfft_set_batch(self._h, n, ctypes.cast(arr_t, ctypes.c_void_p), ctypes.cast(arr_s, ctypes.c_void_p),
+                                       ctypes.cast(arr_o, ctypes.c_void_p), _ptr(peaks_xy), _ptr(values)))
+
+    def FwdImageInvFFTBatch(self, image_to_search, pre_op=None, intra_op=None, post_op=None) -> None:
+        """Extension: every correlation of the batch, asynchronously."""
+        self._bind_torch_stream()
+        _check(lib().fastfft_fwd_image_inv_batch(self._h, _ptr(image_to_search), pre_op or Noop(), intra_op or ConjMul(),
+                                                 post_op or Noop()))
 
     def Wait(self) -> None:
         _check(lib().fastfft_wait(self._h))

@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -79,6 +82,25 @@ struct fastfft_plan_s {
     std::vector<Stage> stages[3];
     bool               stages_built[3] = {false, false, false};
     float2*            tw              = nullptr; // twiddle table of ws_device
+    // batch of correlations set by fastfft_set_batch: one entry per template, and
+    // the xcorr stage list of every distinct template size
+    struct BatchItem {
+        const void* in;
+        void*       out;
+        int         x, y;
+    };
+    std::vector<BatchItem>                       batch;
+    int*                                         batch_peaks  = nullptr; // device int[2n] or null
+    float*                                       batch_values = nullptr; // device float[n] or null
+    std::map<std::pair<int, int>, std::vector<Stage>> batch_stages;
+    // batched kernels (when they cover the plan's stages): chunks of batch_chunk
+    // templates share one launch per stage; batch_dev holds the A and B slots of
+    // a chunk, the argmax keys and the per-template launch descriptors
+    int           batch_chunk = 0; // 0: per-template launches
+    void*         batch_dev   = nullptr;
+    RowBatchItem* batch_r2c   = nullptr;
+    XcorrItem*    batch_xc    = nullptr;
+    RowBatchItem* batch_c2r   = nullptr;
 };
 
 static int op_index(const char* op) {
@@ -95,8 +117,21 @@ static int op_index(const char* op) {
 
 static const char* kOpNames[3] = {"fwd", "inv", "xcorr"};
 
+static void free_batch_device(fastfft_plan p) {
+    if ( p->batch_dev )
+        (void)hipFree(p->batch_dev);
+    p->batch_dev   = nullptr;
+    p->batch_r2c   = nullptr;
+    p->batch_xc    = nullptr;
+    p->batch_c2r   = nullptr;
+    p->batch_chunk = 0;
+}
+
 static void invalidate(fastfft_plan p) {
     p->st.validated = false;
+    p->batch.clear( );
+    p->batch_stages.clear( );
+    free_batch_device(p);
     for ( int i = 0; i < 3; i++ ) {
         p->stages_built[i] = false;
         p->stages[i].clear( );
@@ -163,6 +198,11 @@ static fastfft_status op_kind_check(const fastfft_op& op, bool allow_scale, bool
     return fail(FASTFFT_INVALID_ARGUMENT, std::string("operator kind ") + std::to_string(op.kind) + " is not valid as " + which);
 }
 
+static int env_chunk( ) {
+    const char* v = std::getenv("FASTFFT_BATCH_CHUNK");
+    return v ? std::atoi(v) : 0;
+}
+
 static float op_scale(const fastfft_op& op) {
     return (op.kind == FASTFFT_OP_SCALE || op.kind == FASTFFT_OP_CONJ_MUL_THEN_SCALE) ? op.scale : 1.f;
 }
@@ -183,18 +223,27 @@ static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stag
     return ok( );
 }
 
+static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
+                                     void* out, const void* image, float pre, float intra, float post, int first = 0,
+                                     int last = -1, const ArgmaxSlots* amax = nullptr, bool* amax_fused = nullptr);
+
 static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, const void* image, float pre, float intra,
                                  float post, int first = 0, int last = -1, const ArgmaxSlots* amax = nullptr,
                                  bool* amax_fused = nullptr) {
     const std::vector<Stage>* sp = nullptr;
     if ( auto rc = get_stages(p, opi, &sp) )
         return rc;
-    const std::vector<Stage>& stages = *sp;
+    return run_stage_list(p, *sp, kOpNames[opi], in, out, image, pre, intra, post, first, last, amax, amax_fused);
+}
+
+static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
+                                     void* out, const void* image, float pre, float intra, float post, int first, int last,
+                                     const ArgmaxSlots* amax, bool* amax_fused) {
     if ( last < 0 )
         last = (int)stages.size( ) - 1;
     if ( first < 0 || last >= (int)stages.size( ) || first > last )
         return fail(FASTFFT_INVALID_ARGUMENT,
-                    "stage range outside the " + std::to_string(stages.size( )) + " stages of " + kOpNames[opi]);
+                    "stage range outside the " + std::to_string(stages.size( )) + " stages of " + opname);
     float2*        tw = nullptr;
     fastfft_status rc = ensure_device_state(p, &tw);
     if ( rc )
@@ -286,6 +335,172 @@ static fastfft_status argmax_scratch(fastfft_plan plan, ArgmaxScratch& a) {
     return ok( );
 }
 
+// ---------------------------------------------------------------- batched correlation
+// Chunk sizes chosen by timing, per batch shape, for the life of the process.
+static std::mutex                 g_chunk_mutex;
+static std::map<std::string, int> g_chunk_cache;
+
+static size_t slot_elems_a(const Stage& r2c) { return (size_t)r2c.nlines * (size_t)r2c.om.s1; }
+static size_t slot_elems_b(const Stage& c2r) { return (size_t)c2r.nlines * (size_t)c2r.im.s1; }
+
+// Device block for chunks of C templates: A slots, B slots, argmax keys, then the
+// descriptor arrays of all n templates (filled by fill_batch_items).
+static size_t batch_dev_bytes(const std::vector<Stage>& st, int C, int n) {
+    return sizeof(float2) * (size_t)C * (slot_elems_a(st[0]) + slot_elems_b(st[2])) +
+           sizeof(unsigned long long) * (size_t)C * kArgmaxPartials + (sizeof(RowBatchItem) * 2 + sizeof(XcorrItem)) * (size_t)n;
+}
+
+static fastfft_status fill_batch_items(fastfft_plan p, const std::vector<Stage>& st, int C) {
+    const int n     = (int)p->batch.size( );
+    char*     base  = static_cast<char*>(p->batch_dev);
+    float2*   slotA = reinterpret_cast<float2*>(base);
+    float2*   slotB = slotA + (size_t)C * slot_elems_a(st[0]);
+    auto*     keys  = reinterpret_cast<unsigned long long*>(slotB + (size_t)C * slot_elems_b(st[2]));
+    p->batch_r2c    = reinterpret_cast<RowBatchItem*>(keys + (size_t)C * kArgmaxPartials);
+    p->batch_c2r    = p->batch_r2c + n;
+    p->batch_xc     = reinterpret_cast<XcorrItem*>(p->batch_c2r + n);
+    std::vector<RowBatchItem> r2c(n), c2r(n);
+    std::vector<XcorrItem>    xc(n);
+    for ( int i = 0; i < n; i++ ) {
+        const auto& it = p->batch[i];
+        const int   sl = i % C;
+        float2*     a  = slotA + (size_t)sl * slot_elems_a(st[0]);
+        float2*     b  = slotB + (size_t)sl * slot_elems_b(st[2]);
+        r2c[i]         = RowBatchItem{it.in, a, nullptr, 2L * (it.x / 2 + 1), it.x, it.y};
+        xc[i]          = XcorrItem{a, b, it.y, 0};
+        c2r[i]         = RowBatchItem{b, it.out, p->batch_peaks ? keys + (size_t)sl * kArgmaxPartials : nullptr, 0, 0, 0};
+    }
+    hipError_t e = hipMemcpy(p->batch_r2c, r2c.data( ), sizeof(RowBatchItem) * n, hipMemcpyHostToDevice);
+    if ( e == hipSuccess )
+        e = hipMemcpy(p->batch_c2r, c2r.data( ), sizeof(RowBatchItem) * n, hipMemcpyHostToDevice);
+    if ( e == hipSuccess )
+        e = hipMemcpy(p->batch_xc, xc.data( ), sizeof(XcorrItem) * n, hipMemcpyHostToDevice);
+    return e == hipSuccess ? ok( ) : hip_fail(e, "batch descriptors");
+}
+
+// Launches every chunk of the batch: R2C rows, fused columns, C2R rows (+ argmax keys).
+static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>& st, const void* image, float pre,
+                                       float intra, float post, const float2* tw) {
+    const int   n    = (int)p->batch.size( );
+    const int   C    = p->batch_chunk;
+    const bool  half = p->st.input_type == FASTFFT_INPUT_F16;
+    const RealT rt   = half ? RealT::f16 : RealT::f32;
+    auto*       keys = reinterpret_cast<unsigned long long*>(p->batch_r2c) - (size_t)C * kArgmaxPartials;
+    for ( int c0 = 0; c0 < n; c0 += C ) {
+        const int  cnt = std::min(C, n - c0);
+        hipError_t e   = launch_r2c_rows_batch(st[0], p->batch_r2c + c0, cnt, st[0].nlines, rt, pre, tw, p->stream);
+        if ( e == hipSuccess ) {
+            StageIO io;
+            io.img     = image;
+            io.img_ct  = p->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+            io.scale   = intra;
+            io.xitems  = p->batch_xc + c0;
+            io.nxitems = cnt;
+            e          = launch_xcorr(st[1], io, tw, p->stream);
+        }
+        if ( e == hipSuccess )
+            e = launch_c2r_rows_batch(st[2], p->batch_c2r + c0, cnt, rt, post, tw, p->stream);
+        if ( e == hipSuccess && p->batch_peaks )
+            e = launch_argmax_keys(keys, st[2].nlines, st[2].M, p->batch_values + c0, p->batch_peaks + 2 * c0, p->stream, cnt,
+                                   kArgmaxPartials);
+        if ( e != hipSuccess )
+            return hip_fail(e, "batched correlation launch");
+    }
+    return ok( );
+}
+
+// Picks the chunk size: FASTFFT_BATCH_CHUNK if set, else the fastest of
+// 1, 2, 4, ... templates per launch on this batch (timed on a zero image), cached
+// per batch shape.  Allocates batch_dev for the chosen size and fills the descriptors.
+static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stage>& st) {
+    const int n = (int)p->batch.size( );
+    float2*   tw = nullptr;
+    if ( auto rc = ensure_device_state(p, &tw) )
+        return rc;
+    std::vector<std::pair<int, int>> sizes;
+    for ( const auto& it : p->batch )
+        sizes.push_back({it.x, it.y});
+    std::sort(sizes.begin( ), sizes.end( ));
+    std::string key = std::to_string(st[1].N) + "/" + std::to_string(st[1].P) + "/" + std::to_string(st[0].N) + "/" +
+                      std::to_string(st[2].M) + "x" + std::to_string(st[2].nlines) + "/" + std::to_string(st[1].o) + "/" +
+                      std::to_string(p->st.input_type) + std::to_string(p->st.image_type) + "/" + std::to_string(p->batch_peaks != nullptr);
+    for ( const auto& s : sizes )
+        key += ":" + std::to_string(s.first) + "x" + std::to_string(s.second);
+    int chosen = env_chunk( );
+    if ( chosen <= 0 ) {
+        std::lock_guard<std::mutex> lock(g_chunk_mutex);
+        auto                        f = g_chunk_cache.find(key);
+        if ( f != g_chunk_cache.end( ) )
+            chosen = f->second;
+    }
+    if ( chosen <= 0 ) {
+        // candidates up to the batch size and about 1 GiB of slots
+        const size_t per = sizeof(float2) * (slot_elems_a(st[0]) + slot_elems_b(st[2]));
+        std::vector<int> cand;
+        for ( int C = 1; C <= std::min(n, 64); C *= 2 )
+            if ( C == 1 || per * (size_t)C <= (size_t(1) << 30) )
+                cand.push_back(C);
+        const int cmax = cand.back( );
+        free_batch_device(p);
+        hipError_t e = hipMalloc(&p->batch_dev, batch_dev_bytes(st, cmax, n));
+        void*      img = nullptr;
+        const size_t img_bytes = (p->st.image_type == FASTFFT_IMAGE_C32H ? 4 : 8) * (size_t)st[1].gm.s1 * st[1].nlines;
+        if ( e == hipSuccess )
+            e = hipMalloc(&img, img_bytes);
+        if ( e == hipSuccess )
+            e = hipMemset(img, 0, img_bytes);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if ( e == hipSuccess )
+            e = hipEventCreate(&e0);
+        if ( e == hipSuccess )
+            e = hipEventCreate(&e1);
+        float best = 0.f;
+        chosen     = 1;
+        for ( size_t ci = 0; e == hipSuccess && ci < cand.size( ); ci++ ) {
+            p->batch_chunk = cand[ci];
+            if ( auto rc = fill_batch_items(p, st, cand[ci]) ) {
+                e = hipErrorUnknown;
+                break;
+            }
+            float ms = 0.f;
+            for ( int rep = 0; rep < 2 && e == hipSuccess; rep++ ) {
+                e = hipEventRecord(e0, p->stream);
+                if ( e == hipSuccess && run_batch_chunks(p, st, img, 1.f, 1.f, 1.f, tw) != FASTFFT_SUCCESS )
+                    e = hipErrorUnknown;
+                if ( e == hipSuccess )
+                    e = hipEventRecord(e1, p->stream);
+                if ( e == hipSuccess )
+                    e = hipEventSynchronize(e1);
+                if ( e == hipSuccess )
+                    e = hipEventElapsedTime(&ms, e0, e1);
+            }
+            if ( e == hipSuccess && (ci == 0 || ms < best) ) {
+                best   = ms;
+                chosen = cand[ci];
+            }
+        }
+        if ( e0 )
+            (void)hipEventDestroy(e0);
+        if ( e1 )
+            (void)hipEventDestroy(e1);
+        if ( img )
+            (void)hipFree(img);
+        free_batch_device(p);
+        if ( e != hipSuccess )
+            return hip_fail(e, "batch chunk tuning");
+        std::lock_guard<std::mutex> lock(g_chunk_mutex);
+        g_chunk_cache[key] = chosen;
+    }
+    free_batch_device(p);
+    hipError_t e = hipMalloc(&p->batch_dev, batch_dev_bytes(st, chosen, n));
+    if ( e != hipSuccess ) {
+        p->batch_dev = nullptr;
+        return hip_fail(e, "hipMalloc(batch)");
+    }
+    p->batch_chunk = chosen;
+    return fill_batch_items(p, st, chosen);
+}
+
 extern "C" {
 
 fastfft_status fastfft_create(fastfft_plan* plan, int input_type, int image_type, int rank) {
@@ -315,6 +530,7 @@ fastfft_status fastfft_destroy(fastfft_plan plan) {
         (void)hipFree(plan->ws);
     if ( plan->argmax_scratch )
         (void)hipFree(plan->argmax_scratch);
+    free_batch_device(plan);
     delete plan;
     return ok( );
 }
@@ -480,6 +696,123 @@ fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const v
         if ( e != hipSuccess )
             return hip_fail(e, "argmax");
     }
+    return ok( );
+}
+
+fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* templates, const int* sizes_xy,
+                                 void* const* outputs, int* peaks_xy, float* values) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( n < 0 || (n > 0 && (! templates || ! sizes_xy || ! outputs)) || ((peaks_xy == nullptr) != (values == nullptr)) )
+        return fail(FASTFFT_INVALID_ARGUMENT, "set_batch: bad count, null array, or only one of peaks_xy / values");
+    if ( auto rc = ensure_validated(plan) )
+        return rc;
+    if ( plan->st.transform_dimension != 2 )
+        return fail(FASTFFT_UNSUPPORTED, "batched FwdImageInvFFT is implemented for 2D plans");
+    const Dims& in = plan->st.fwd_in;
+    std::vector<fastfft_plan_s::BatchItem>            items;
+    std::map<std::pair<int, int>, std::vector<Stage>> lists;
+    for ( int i = 0; i < n; i++ ) {
+        const int x = sizes_xy[2 * i], y = sizes_xy[2 * i + 1];
+        if ( ! templates[i] || ! outputs[i] )
+            return fail(FASTFFT_INVALID_ARGUMENT, "set_batch: null template or output pointer at " + std::to_string(i));
+        if ( x < 1 || y < 1 || x > in.x || y > in.y )
+            return fail(FASTFFT_INVALID_ARGUMENT, "set_batch: template " + std::to_string(i) + " is " + std::to_string(x) + "x" +
+                                                          std::to_string(y) + ", outside 1.." + std::to_string(in.x) + "x" +
+                                                          std::to_string(in.y) + " (the plan's forward input size)");
+        items.push_back({templates[i], outputs[i], x, y});
+        const auto key = std::make_pair(x, y);
+        if ( lists.count(key) )
+            continue;
+        // the plan with this template size as its forward input: same padded size,
+        // window and types, so the workspace of the plan covers it
+        PlanState ps = plan->st;
+        ps.fwd_in    = Dims{x, y, 1, 0};
+        PlanError pe = validate(ps);
+        std::vector<Stage> st;
+        if ( ! pe.code )
+            pe = build_stages(ps, "xcorr", st);
+        if ( pe.code )
+            return fail(pe.code, "set_batch: template " + std::to_string(i) + ": " + pe.msg);
+        lists[key] = std::move(st);
+    }
+    free_batch_device(plan);
+    plan->batch        = std::move(items);
+    plan->batch_stages = std::move(lists);
+    plan->batch_peaks  = peaks_xy;
+    plan->batch_values = values;
+    // one launch per stage for chunks of templates when the fast kernels cover the plan's stages
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(plan, 2, &sp) )
+        return rc;
+    const CpxT img_ct = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+    if ( n > 0 && sp->size( ) == 3 && batch_kernels_cover((*sp)[0], (*sp)[1], (*sp)[2], img_ct) ) {
+        if ( auto rc = setup_batch_kernels(plan, *sp) ) {
+            plan->batch.clear( );
+            plan->batch_stages.clear( );
+            return rc;
+        }
+    }
+    return ok( );
+}
+
+fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_to_search, fastfft_op pre_op,
+                                           fastfft_op intra_op, fastfft_op post_op) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! image_to_search )
+        return fail(FASTFFT_INVALID_ARGUMENT, "null image_to_search pointer");
+    if ( auto rc = op_kind_check(pre_op, true, false, "the FwdImageInvFFT pre op") )
+        return rc;
+    if ( intra_op.kind != FASTFFT_OP_CONJ_MUL && intra_op.kind != FASTFFT_OP_CONJ_MUL_THEN_SCALE )
+        return fail(FASTFFT_INVALID_ARGUMENT, "FwdImageInvFFT needs an intra op (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
+    if ( auto rc = op_kind_check(post_op, true, false, "the FwdImageInvFFT post op") )
+        return rc;
+    if ( auto rc = ensure_validated(plan) )
+        return rc;
+    if ( plan->batch_chunk > 0 && reinterpret_cast<std::uintptr_t>(image_to_search) % 16 == 0 ) {
+        const std::vector<Stage>* sp = nullptr;
+        if ( auto rc = get_stages(plan, 2, &sp) )
+            return rc;
+        float2* tw = nullptr;
+        if ( auto rc = ensure_device_state(plan, &tw) )
+            return rc;
+        if ( auto rc = run_batch_chunks(plan, *sp, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), tw) )
+            return rc;
+        if ( ! plan->batch.empty( ) ) {
+            plan->last_buffer     = plan->batch.back( ).out;
+            plan->stage_completed = 7;
+        }
+        return ok( );
+    }
+    ArgmaxScratch a;
+    if ( plan->batch_peaks )
+        if ( auto rc = argmax_scratch(plan, a) )
+            return rc;
+    const Dims& o = plan->st.inv_out;
+    for ( size_t i = 0; i < plan->batch.size( ); i++ ) {
+        const auto& it = plan->batch[i];
+        const auto& st = plan->batch_stages.at(std::make_pair(it.x, it.y));
+        ArgmaxSlots slots = a.fused;
+        bool        fused = false;
+        if ( plan->batch_peaks ) {
+            slots.value = plan->batch_values + i;
+            slots.xy    = plan->batch_peaks + 2 * i;
+        }
+        if ( auto rc = run_stage_list(plan, st, "xcorr", const_cast<void*>(it.in), it.out, image_to_search,
+                                      op_scale(pre_op), op_scale(intra_op), op_scale(post_op), 0, -1,
+                                      plan->batch_peaks ? &slots : nullptr, &fused) )
+            return rc;
+        if ( plan->batch_peaks && ! fused ) {
+            hipError_t e = launch_argmax(it.out, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, o.x,
+                                         o.y, 2 * o.w, a.pv, a.pi, slots.value, slots.xy, plan->stream);
+            if ( e != hipSuccess )
+                return hip_fail(e, "argmax");
+        }
+        plan->last_buffer = it.out;
+    }
+    if ( ! plan->batch.empty( ) )
+        plan->stage_completed = 7;
     return ok( );
 }
 

@@ -1,0 +1,104 @@
+// Full-length forward line transforms (no size change) with one workgroup of
+// Q wavefronts per line: the R2C x-pass and the C2C y-pass of an unpadded
+// FwdFFT (the reference's block_fft_kernel_R2C_NONE_XY and C2C_NONE,
+// src/fastfft/FastFFT.cu:1055-1077, :1348-1367), used for the micrograph
+// spectrum of the correlation path.
+//
+// N = P*Q is split decimation-in-frequency style:
+//   y_q[m]      = sum_s x[s P + m] W_Q^{s q}          (radix-Q pass over the line in LDS)
+//   X[Q k' + q] = FFT_P( y_q[m] W_N^{m q} )[k']         (wave q, register Stockham)
+// The line is read with coalesced loads into LDS.  A unit-stride output line is
+// written through LDS with 16-byte stores; a strided one (the transposed x-pass
+// output) one value per output line, with each XCD owning a contiguous range
+// of lines so that the partial lines meet in one L2.
+#pragma once
+
+#include "rows_wave.hpp"
+
+namespace fastfft_amd {
+
+template <int P, int Q, typename TIn, bool REAL>
+__global__ void __launch_bounds__(64 * Q)
+        k_fwd_full_wave(const TIn* __restrict__ in, LineMap im, float2* __restrict__ out, LineMap om, int nout, int nlines,
+                        float pre_scale, const float2* __restrict__ tw) {
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q;
+    using FFT       = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
+    static_assert(P % NT == 0 || NT % P == 0, "whole groups of the radix-Q pass per thread");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_twn[NT];
+    __shared__ float  s_real[REAL ? N : 1];
+    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t    = threadIdx.x % T;
+    const int     line = xcd_row(blockIdx.x, nlines);
+    const float2* twm  = multi_twiddles(tw);
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
+    // ---- the line into LDS (natural order)
+    const TIn* lp = in + im.base(line);
+    if constexpr ( REAL ) {
+        for ( int n = threadIdx.x; n < N; n += NT )
+            s_real[n] = load_real(lp, n * im.es) * pre_scale;
+    }
+    else {
+        for ( int n = threadIdx.x; n < N; n += NT ) {
+            const float2 z = load_cpx(lp, n * im.es);
+            lds[n]         = make_float2(z.x * pre_scale, z.y * pre_scale);
+        }
+    }
+    __syncthreads( );
+    // ---- radix-Q pass: thread m holds x[s P + m], s < Q, and leaves y_q[m] at q P + m
+    for ( int m = threadIdx.x; m < P; m += NT ) {
+        float2 u[Q];
+#pragma unroll
+        for ( int s = 0; s < Q; s++ ) {
+            if constexpr ( REAL )
+                u[s] = make_float2(s_real[s * P + m], 0.f);
+            else
+                u[s] = lds[s * P + m];
+        }
+        SmallDFT<Q, -1>::run(u);
+#pragma unroll
+        for ( int s = 0; s < Q; s++ )
+            lds[s * P + m] = u[s];
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_p(t, s_twp);
+    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    float2       v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        v[i] = cmul(cmul(lds[q * P + t + T * i], twm[N + ((T * i * q) & (N - 1))]), lw);
+    __syncthreads( );
+    fft.template run<-1>(v, lds + q * FFT::SLOTS, t);
+    float2* op = out + om.base(line);
+    if ( om.es == 1 ) {
+        // natural order through LDS, then 16-byte stores (op is 16-byte aligned)
+        __syncthreads( );
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            lds[dec_slot<P, Q>(Q * (t + T * i) + q)] = v[i];
+        __syncthreads( );
+        for ( int m = threadIdx.x; 2 * m < nout; m += NT ) {
+            const float2 u0 = lds[dec_slot<P, Q>(2 * m)];
+            if ( 2 * m + 1 < nout ) {
+                const float2 u1                  = lds[dec_slot<P, Q>(2 * m + 1)];
+                reinterpret_cast<float4*>(op)[m] = make_float4(u0.x, u0.y, u1.x, u1.y);
+            }
+            else {
+                op[2 * m] = u0;
+            }
+        }
+    }
+    else {
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int k = Q * (t + T * i) + q;
+            if ( k < nout )
+                op[k * om.es] = v[i];
+        }
+    }
+}
+
+} // namespace fastfft_amd

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "items.hpp"
 #include "plan.hpp"
 
 namespace fastfft_amd {
@@ -32,7 +33,24 @@ struct StageIO {
     // kernel supports it, and set *amax_fused (left unchanged otherwise)
     const ArgmaxSlots* amax       = nullptr;
     bool*              amax_fused = nullptr;
+    // xcorr only: correlate with the templates xitems[0 .. nxitems) (device
+    // array of XcorrItem, xcorr_col.hpp) instead of src / dst
+    const void* xitems  = nullptr;
+    int         nxitems = 0;
 };
+
+// Batched correlation over n templates with the stage parameters of `s` (the
+// stages of the largest template); items are device arrays (RowBatchItem,
+// rows_wave.hpp; XcorrItem, xcorr_col.hpp).  Each returns hipErrorNotSupported,
+// before launching anything, when its fast kernel does not cover the stage.
+bool       r2c_rows_batch_covers(const Stage& s);
+bool       c2r_rows_batch_covers(const Stage& s);
+bool       xcorr_col_covers(const Stage& s, CpxT img_ct);
+bool       batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r, CpxT img_ct);
+hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
+                                 const float2* tw, hipStream_t stream);
+hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,
+                                 hipStream_t stream);
 
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
@@ -52,9 +70,10 @@ constexpr int kArgmaxPartials = 512;
 hipError_t    launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, float* partial_v, int* partial_i,
                             float* result_value, int* result_xy, hipStream_t stream);
 
-// Reduces n argmax keys (fft_core.hpp) to the value and (x, y) = (index mod nx, index / nx).
+// Reduces n argmax keys (fft_core.hpp) to the value and (x, y) = (index mod nx, index / nx);
+// with batch > 1, set b reduces keys[b * key_stride ..] into result_value[b], result_xy[2b..].
 hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, float* result_value, int* result_xy,
-                              hipStream_t stream);
+                              hipStream_t stream, int batch = 1, long key_stride = 0);
 
 // LDS bytes per workgroup of a line stage with sub-transform size P.
 size_t stage_lds_bytes(int P);

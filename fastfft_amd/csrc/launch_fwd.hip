@@ -1,6 +1,7 @@
 // Forward line stages: R2C x-pass (real input) and C2C y/z passes.
 #include <cstdint>
 
+#include "full_wave.hpp"
 #include "launch_common.hpp"
 #include "rows_wave.hpp"
 
@@ -24,8 +25,58 @@ template <int P, int Q, typename TIn>
 static hipError_t go_r2c_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     k_r2c_rows_wave<P, Q, TIn><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
             static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<float2*>(io.dst), to_device(s.om), s.nout,
-            s.nlines, io.scale, tw);
+            s.nlines, io.scale, tw, nullptr);
     return hipGetLastError( );
+}
+
+template <int P, int Q, typename TIn>
+static hipError_t go_r2c_wave_batch(const Stage& s, const RowBatchItem* items, int n, int max_rows, float pre,
+                                    const float2* tw, hipStream_t st) {
+    k_r2c_rows_wave<P, Q, TIn><<<dim3((unsigned)max_rows, (unsigned)n), dim3(64 * Q), 0, st>>>(
+            nullptr, to_device(s.im), s.L, nullptr, to_device(s.om), s.nout, max_rows, pre, tw, items);
+    return hipGetLastError( );
+}
+
+// (P, Q) of an R2C row stage that k_r2c_rows_wave covers
+static bool r2c_rows_shape(const Stage& s) {
+    return (s.P == 256 || s.P == 512 || s.P == 1024) && (s.Q == 1 || s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) &&
+           s.P * s.Q <= kTwiddleN;
+}
+
+bool r2c_rows_batch_covers(const Stage& s) {
+    // one template per grid row: single-index lines, unit-stride row-major output with even leading dimension
+    return r2c_rows_shape(s) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1 && s.om.es == 1 && s.om.s1 % 2 == 0;
+}
+
+hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
+                                 const float2* tw, hipStream_t st) {
+    if ( ! r2c_rows_batch_covers(s) )
+        return hipErrorNotSupported;
+    const auto* it   = static_cast<const RowBatchItem*>(items);
+    auto        by_p = [&](auto pc) -> hipError_t {
+        constexpr int P  = decltype(pc)::value;
+        auto          go = [&](auto qc) -> hipError_t {
+            constexpr int Q = decltype(qc)::value;
+            if constexpr ( P * Q > kTwiddleN )
+                return hipErrorNotSupported;
+            else
+                return in_rt == RealT::f32 ? go_r2c_wave_batch<P, Q, float>(s, it, n, max_rows, pre, tw, st)
+                                           : go_r2c_wave_batch<P, Q, __half>(s, it, n, max_rows, pre, tw, st);
+        };
+        switch ( s.Q ) {
+            case 1: return go(std::integral_constant<int, 1>{ });
+            case 2: return go(std::integral_constant<int, 2>{ });
+            case 4: return go(std::integral_constant<int, 4>{ });
+            case 8: return go(std::integral_constant<int, 8>{ });
+            case 16: return go(std::integral_constant<int, 16>{ });
+            default: return hipErrorNotSupported;
+        }
+    };
+    if ( s.P == 256 )
+        return by_p(std::integral_constant<int, 256>{ });
+    if ( s.P == 512 )
+        return by_p(std::integral_constant<int, 512>{ });
+    return by_p(std::integral_constant<int, 1024>{ });
 }
 
 template <typename TIn>
@@ -53,10 +104,57 @@ static hipError_t try_r2c_wave(const Stage& s, const StageIO& io, const float2* 
     return hipErrorNotSupported;
 }
 
+// Full-length lines (L = N, 256 <= N <= 8192) with a float2 output: one
+// workgroup per line, P = 512 (256 for N = 256) and Q = N / P.
+template <int P, int Q>
+static hipError_t go_full_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const dim3 grid((unsigned)s.nlines), block(64 * Q);
+    if ( io.src_real ) {
+        if ( io.src_rt == RealT::f32 )
+            k_fwd_full_wave<P, Q, float, true><<<grid, block, 0, st>>>(static_cast<const float*>(io.src), to_device(s.im),
+                                                                       static_cast<float2*>(io.dst), to_device(s.om),
+                                                                       s.nout, s.nlines, io.scale, tw);
+        else
+            k_fwd_full_wave<P, Q, __half, true><<<grid, block, 0, st>>>(static_cast<const __half*>(io.src), to_device(s.im),
+                                                                        static_cast<float2*>(io.dst), to_device(s.om),
+                                                                        s.nout, s.nlines, io.scale, tw);
+    }
+    else {
+        if ( io.src_ct == CpxT::c64 )
+            k_fwd_full_wave<P, Q, float2, false><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+                                                                         static_cast<float2*>(io.dst), to_device(s.om),
+                                                                         s.nout, s.nlines, io.scale, tw);
+        else
+            k_fwd_full_wave<P, Q, __half2, false><<<grid, block, 0, st>>>(static_cast<const __half2*>(io.src),
+                                                                          to_device(s.im), static_cast<float2*>(io.dst),
+                                                                          to_device(s.om), s.nout, s.nlines, io.scale, tw);
+    }
+    return hipGetLastError( );
+}
+
+static hipError_t try_full_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( s.L != s.N || io.dst_real || io.dst_ct != CpxT::c64 || s.nout > s.N )
+        return hipErrorNotSupported;
+    switch ( s.N ) {
+        case 256: return go_full_wave<256, 1>(s, io, tw, st);
+        case 512: return go_full_wave<512, 1>(s, io, tw, st);
+        case 1024: return go_full_wave<512, 2>(s, io, tw, st);
+        case 2048: return go_full_wave<512, 4>(s, io, tw, st);
+        case 4096: return go_full_wave<512, 8>(s, io, tw, st);
+        case 8192: return go_full_wave<512, 16>(s, io, tw, st);
+        default: return hipErrorNotSupported;
+    }
+}
+
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     // a unit-stride output row is written with 16-byte stores: even line offsets, aligned base
     const bool rows_ok = s.om.es != 1 || (s.om.s1 % 2 == 0 && s.om.s2 % 2 == 0 &&
                                           reinterpret_cast<std::uintptr_t>(io.dst) % 16 == 0);
+    if ( rows_ok && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) && ! env_int("FASTFFT_NO_FULL_LINES", 0) ) {
+        hipError_t e = try_full_wave(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && rows_ok && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = io.src_rt == RealT::f32 ? try_r2c_wave<float>(s, io, tw, st) : try_r2c_wave<__half>(s, io, tw, st);
         if ( e != hipErrorNotSupported )

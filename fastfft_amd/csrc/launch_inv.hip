@@ -25,7 +25,7 @@ static hipError_t go_c2r_wave(const Stage& s, const StageIO& io, const float2* t
     const bool keys = io.amax && s.nlines <= kArgmaxPartials;
     k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
             static_cast<const float2*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
-            s.nlines, io.scale, tw, keys ? io.amax->keys : nullptr);
+            s.nlines, io.scale, tw, keys ? io.amax->keys : nullptr, nullptr);
     hipError_t e = hipGetLastError( );
     if ( e == hipSuccess && keys ) {
         e = launch_argmax_keys(io.amax->keys, s.nlines, s.M, io.amax->value, io.amax->xy, st);
@@ -70,6 +70,66 @@ static hipError_t try_c2r_wave(const Stage& s, const StageIO& io, const float2* 
     if ( s.P == 1024 )
         return by_p(std::integral_constant<int, 1024>{ });
     return hipErrorNotSupported;
+}
+
+template <int P, int Q, int RS, typename TOut>
+static hipError_t go_c2r_wave_batch(const Stage& s, const RowBatchItem* items, int n, float post, const float2* tw,
+                                    hipStream_t st) {
+    k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines, (unsigned)n), dim3(64 * Q), 0, st>>>(
+            nullptr, to_device(s.im), nullptr, to_device(s.om), s.M, s.o, s.nlines, post, tw, nullptr, items);
+    return hipGetLastError( );
+}
+
+// window origins and (P, Q) that k_c2r_rows_wave covers
+static int c2r_rows_rotation(const Stage& s) {
+    if ( s.o % 64 != 0 || ! (s.P == 256 || s.P == 512 || s.P == 1024) || s.P * s.Q > kTwiddleN ||
+         ! (s.Q == 1 || s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) )
+        return -1;
+    const int E  = s.P / 64;
+    const int rs = (((s.o / 64) % E) + E) % E;
+    return rs == 0 || rs == E / 2 ? rs : -1;
+}
+
+bool c2r_rows_batch_covers(const Stage& s) {
+    return c2r_rows_rotation(s) >= 0 && s.nlines <= kArgmaxPartials && s.im.n2 == 1 && s.om.n2 == 1;
+}
+
+hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,
+                                 hipStream_t st) {
+    if ( ! c2r_rows_batch_covers(s) )
+        return hipErrorNotSupported;
+    const auto* it   = static_cast<const RowBatchItem*>(items);
+    const int   rs   = c2r_rows_rotation(s);
+    auto        by_p = [&](auto pc) -> hipError_t {
+        constexpr int P  = decltype(pc)::value;
+        constexpr int E  = P / 64;
+        auto          go = [&](auto qc) -> hipError_t {
+            constexpr int Q = decltype(qc)::value;
+            if constexpr ( P * Q > kTwiddleN ) {
+                return hipErrorNotSupported;
+            }
+            else {
+                if ( out_rt == RealT::f32 )
+                    return rs == 0 ? go_c2r_wave_batch<P, Q, 0, float>(s, it, n, post, tw, st)
+                                   : go_c2r_wave_batch<P, Q, E / 2, float>(s, it, n, post, tw, st);
+                return rs == 0 ? go_c2r_wave_batch<P, Q, 0, __half>(s, it, n, post, tw, st)
+                               : go_c2r_wave_batch<P, Q, E / 2, __half>(s, it, n, post, tw, st);
+            }
+        };
+        switch ( s.Q ) {
+            case 1: return go(std::integral_constant<int, 1>{ });
+            case 2: return go(std::integral_constant<int, 2>{ });
+            case 4: return go(std::integral_constant<int, 4>{ });
+            case 8: return go(std::integral_constant<int, 8>{ });
+            case 16: return go(std::integral_constant<int, 16>{ });
+            default: return hipErrorNotSupported;
+        }
+    };
+    if ( s.P == 256 )
+        return by_p(std::integral_constant<int, 256>{ });
+    if ( s.P == 512 )
+        return by_p(std::integral_constant<int, 512>{ });
+    return by_p(std::integral_constant<int, 1024>{ });
 }
 
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {

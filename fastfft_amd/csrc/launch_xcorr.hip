@@ -33,9 +33,10 @@ template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const int pack   = s.real_end_lines && s.nlines >= 2 && env_int("FASTFFT_XCORR_PACK", 1) ? 1 : 0;
     const int blocks = s.nlines - pack;
+    XcorrItem one{io.src, io.dst, s.L, 0};
     k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks), dim3(64 * Q), 0, st>>>(
-            static_cast<const float2*>(io.src), s.im.s1, s.im.es, s.L, static_cast<const TImg*>(io.img), s.gm.s1,
-            static_cast<float2*>(io.dst), s.om.s1, s.om.es, s.M, s.o, io.scale, tw, s.nlines - 1, pack);
+            one, static_cast<const XcorrItem*>(io.xitems), io.nxitems, s.im.s1, s.im.es, static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M,
+            s.o, io.scale, tw, s.nlines - 1, pack);
     return hipGetLastError( );
 }
 
@@ -69,11 +70,28 @@ static hipError_t try_xcorr_col(const Stage& s, const StageIO& io, const float2*
 
 static bool single_index_lines(const HostLineMap& m) { return m.n2 == 1 && m.s2 == 0; }
 
+bool xcorr_col_covers(const Stage& s, CpxT img_ct) {
+    const int per = img_ct == CpxT::c32h ? 4 : 2;
+    if ( ! (single_index_lines(s.im) && single_index_lines(s.om) && single_index_lines(s.gm) && s.gm.es == 1 &&
+            s.gm.s1 % per == 0) )
+        return false;
+    if ( ! (s.P == 256 || s.P == 512 || s.P == 1024) || s.o % 64 != 0 || s.P * s.Q > kTwiddleN ||
+         ! (s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) )
+        return false;
+    const int E  = s.P / 64;
+    const int rs = (((s.o / 64) % E) + E) % E;
+    return rs == 0 || rs == E / 2;
+}
+
+bool batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r, CpxT img_ct) {
+    return r2c.kind == StageKind::fwd_real && xcorr.kind == StageKind::xcorr && c2r.kind == StageKind::inv_c2r &&
+           r2c_rows_batch_covers(r2c) && xcorr_col_covers(xcorr, img_ct) && c2r_rows_batch_covers(c2r) &&
+           ! env_int("FASTFFT_NO_WAVE_XCORR", 0) && ! env_int("FASTFFT_NO_WAVE_ROWS", 0);
+}
+
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    // 16-byte image loads: contiguous columns, aligned image base and column starts
-    const int  per    = io.img_ct == CpxT::c32h ? 4 : 2;
-    const bool col_ok = single_index_lines(s.im) && single_index_lines(s.om) && single_index_lines(s.gm) &&
-                        s.gm.es == 1 && reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 && s.gm.s1 % per == 0 &&
+    // 16-byte image loads: aligned image base (the column starts are checked with the shape)
+    const bool col_ok = xcorr_col_covers(s, io.img_ct) && reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 &&
                         ! env_int("FASTFFT_NO_WAVE_XCORR", 0);
     if ( col_ok ) {
         hipError_t e    = hipErrorNotSupported;
@@ -90,6 +108,9 @@ hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hip
         if ( e != hipErrorNotSupported )
             return e;
     }
+    // the generic kernel correlates one template per launch
+    if ( io.xitems )
+        return hipErrorNotSupported;
     return dispatch_P(s.P, [&](auto pc) -> hipError_t {
         constexpr int P = decltype(pc)::value;
         if ( io.img_ct == CpxT::c64 )

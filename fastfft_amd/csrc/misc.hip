@@ -104,8 +104,12 @@ __global__ void __launch_bounds__(kArgmaxThreads)
 
 // First maximum of n keys (value order, then smaller index).
 __global__ void __launch_bounds__(kArgmaxThreads)
-        k_argmax_keys(const unsigned long long* __restrict__ keys, int n, int nx, float* __restrict__ rv, int* __restrict__ rxy) {
+        k_argmax_keys(const unsigned long long* __restrict__ keys, int n, int nx, float* __restrict__ rv, int* __restrict__ rxy,
+                      long key_stride) {
     __shared__ unsigned long long s_best;
+    keys += blockIdx.x * key_stride;
+    rv += blockIdx.x;
+    rxy += 2 * blockIdx.x;
     unsigned long long            best = 0;
     for ( int i = threadIdx.x; i < n; i += kArgmaxThreads ) {
         const unsigned long long k = keys[i];
@@ -131,8 +135,8 @@ __global__ void __launch_bounds__(kArgmaxThreads)
 }
 
 hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, float* result_value, int* result_xy,
-                              hipStream_t stream) {
-    k_argmax_keys<<<1, kArgmaxThreads, 0, stream>>>(keys, n, nx, result_value, result_xy);
+                              hipStream_t stream, int batch, long key_stride) {
+    k_argmax_keys<<<batch, kArgmaxThreads, 0, stream>>>(keys, n, nx, result_value, result_xy, key_stride);
     return hipGetLastError( );
 }
 

@@ -24,6 +24,7 @@
 // in one L2.
 #pragma once
 
+#include "items.hpp"
 #include "line_kernels.hpp"
 #include "wave_fft.hpp"
 
@@ -59,10 +60,12 @@ __device__ __forceinline__ void stage_row_twiddles(float2* s_twp, float2* s_twn,
         s_twn[k] = twm[N + k];
 }
 
+// items == nullptr: one row per workgroup of in / out; else row blockIdx.x of
+// template items[blockIdx.y] (rows past its count return at once).
 template <int P, int Q, typename TIn>
 __global__ void __launch_bounds__(64 * Q)
         k_r2c_rows_wave(const TIn* __restrict__ in, LineMap im, int L, float2* __restrict__ out, LineMap om, int nout,
-                        int nrows, float pre_scale, const float2* __restrict__ tw) {
+                        int nrows, float pre_scale, const float2* __restrict__ tw, const RowBatchItem* __restrict__ items) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q;
     using FFT       = WaveFFT<P>;
     static_assert(Q * FFT::SLOTS >= N, "a row must fit the exchange buffers");
@@ -74,6 +77,15 @@ __global__ void __launch_bounds__(64 * Q)
     const int     t   = threadIdx.x % T;
     const int     row = xcd_row(blockIdx.x, nrows);
     const float2* twm = multi_twiddles(tw);
+    if ( items ) {
+        const RowBatchItem it = items[blockIdx.y];
+        if ( row >= it.rows )
+            return;
+        in    = static_cast<const TIn*>(it.in);
+        out   = static_cast<float2*>(it.out);
+        im.s1 = it.in_ld;
+        L     = it.L;
+    }
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
     const TIn* rp = in + im.base(row);
     for ( int n = threadIdx.x; n < L; n += NT )
@@ -139,11 +151,13 @@ __device__ __forceinline__ unsigned long long block_max_key(unsigned long long k
 
 // row_keys == nullptr: no argmax; else workgroup b stores the largest argmax key
 // (fft_core.hpp) of its row, index row * M + j, to row_keys[b] for a one-workgroup
-// reduction after the launch.
+// reduction after the launch.  items != nullptr: template items[blockIdx.y]
+// supplies in, out and row_keys.
 template <int P, int Q, int RS, typename TOut>
 __global__ void __launch_bounds__(64 * Q)
         k_c2r_rows_wave(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o, int nrows,
-                        float post_scale, const float2* __restrict__ tw, unsigned long long* __restrict__ row_keys) {
+                        float post_scale, const float2* __restrict__ tw, unsigned long long* __restrict__ row_keys,
+                        const RowBatchItem* __restrict__ items) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
     using FFT       = WaveFFT<P>;
     static_assert(Q * FFT::SLOTS >= N, "a row must fit the exchange buffers");
@@ -154,6 +168,12 @@ __global__ void __launch_bounds__(64 * Q)
     const int     t   = threadIdx.x % T;
     const int     row = xcd_row(blockIdx.x, nrows);
     const float2* twm = multi_twiddles(tw);
+    if ( items ) {
+        const RowBatchItem it = items[blockIdx.y];
+        in                    = static_cast<const float2*>(it.in);
+        out                   = static_cast<TOut*>(it.out);
+        row_keys              = it.keys;
+    }
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
     const float2* cp     = in + im.base(row);
     const bool    staged = im.es == 1;

@@ -58,21 +58,24 @@ struct ImgStage<__half2> {
 };
 
 // Grid: nlines - 1 workgroups when pack_ends (columns 0 and last_line share
-// workgroup 0), else nlines.  Element n of column c: a[c*lda + n*a_es],
-// img[c*ldg + n] (img and img + ldg 16-byte aligned), out[c*ldo + n*o_es].
-// With a_es, o_es > 1 (row-major template spectrum and output, as the x-pass
-// kernels prefer) the template gather and the output scatter are the only
-// strided accesses.
+// workgroup 0), else nlines.  img[c*ldg + n] is element n of image column c
+// (img and img + ldg 16-byte aligned).  The workgroup correlates its column
+// with every template of the launch: `one`, or items[0 .. nitems) when items is
+// not null.  The image column is read from memory once per launch and kept in
+// registers (in LDS for the packed end columns) for all templates.  With
+// a_es, o_es > 1 (row-major template spectra and outputs, as the x-pass kernels
+// prefer) the template gather and the output scatter are the only strided
+// accesses.
 //
 // Every table a wave needs is copied into LDS once per workgroup with
 // coalesced loads (W_P^k for the Stockham passes, W_N^k for k < 64 Q for the
-// per-lane input and output twiddles, the template column), and the
-// wave-uniform twiddles W_N^{64 i q} are scalar loads, so the only vector
-// memory traffic is the coalesced column data.
+// per-lane input and output twiddles), and the wave-uniform twiddles
+// W_N^{64 i q} are scalar loads, so the only vector memory traffic is the
+// column data.
 template <int P, int Q, int RS, typename TImg>
 __global__ void __launch_bounds__(64 * Q)
-        k_xcorr_col(const float2* __restrict__ a, long lda, long a_es, int L, const TImg* __restrict__ img, long ldg,
-                    float2* __restrict__ out, long ldo, long o_es, int M, int o, float intra_scale,
+        k_xcorr_col(XcorrItem one, const XcorrItem* __restrict__ items, int nitems, long lda, long a_es,
+                    const TImg* __restrict__ img, long ldg, long ldo, long o_es, int M, int o, float intra_scale,
                     const float2* __restrict__ tw, int last_line, int pack_ends) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, NW = T * Q;
     constexpr int PER = ImgStage<TImg>::PER;
@@ -91,18 +94,14 @@ __global__ void __launch_bounds__(64 * Q)
     const long     col  = xcd_row(blockIdx.x, gridDim.x);
     const bool     ends = pack_ends && col == 0;
     const float2*  twm  = multi_twiddles(tw);
+    if ( ! items )
+        nitems = 1;
 
-    // ---- stage: twiddle tables, template column, image column
+    // ---- stage: twiddle tables, image column
     for ( int k = threadIdx.x; k < P; k += NT )
         s_twp[k] = twm[P + k];
     for ( int k = threadIdx.x; k < NW; k += NT )
         s_twn[k] = twm[N + k];
-    {
-        const float2* ac = a + col * lda;
-        const float2* a1 = a + (long)last_line * lda;
-        for ( int n = threadIdx.x; n < L; n += NT )
-            s_tpl[n] = ends ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
-    }
     if ( ! ends ) {
         const TImg* gc = img + col * ldg;
 #pragma unroll
@@ -118,85 +117,100 @@ __global__ void __launch_bounds__(64 * Q)
     __syncthreads( );
     FFT fft;
     fft.init_p(t, s_twp);
-    // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
-    const float2 lw = s_twn[t * q];
-    float2       v[E];
-#pragma unroll
-    for ( int i = 0; i < E; i++ ) {
-        const int    n = t + T * i;
-        const float2 x = n < L ? s_tpl[n] : make_float2(0.f, 0.f);
-        v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
-    }
-    float2 g[E];
+    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    float2       g[E];
     if ( ! ends ) {
 #pragma unroll
         for ( int i = 0; i < E; i++ )
             g[i] = lds[dec_slot<P, Q>(Q * (t + T * i) + q)];
     }
-    __syncthreads( );
-    float2* wl = lds + q * FFT::SLOTS;
-    fft.template run<-1>(v, wl, t);
-    if ( ! ends ) {
-#pragma unroll
-        for ( int i = 0; i < E; i++ )
-            v[i] = cmulc(g[i], v[i]); // img * conj(X)
-    }
-    else {
-        // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
-        // k'' = (P - k') mod P when q = 0, P - 1 - k' otherwise
-#pragma unroll
-        for ( int i = 0; i < E; i++ )
-            wl[FFT::Layout::slot(t + T * i)] = v[i];
-        __syncthreads( );
-        const int qp = (Q - q) % Q;
-        float2    zm[E];
-#pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int kp  = t + T * i;
-            const int kpp = q == 0 ? (P - kp) & (P - 1) : P - 1 - kp;
-            zm[i]         = lds[qp * FFT::SLOTS + FFT::Layout::slot(kpp)];
-        }
-        __syncthreads( );
-        const TImg* g0 = img;
-        const TImg* g1 = img + (long)last_line * ldg;
-#pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int    k  = Q * (t + T * i) + q;
-            const int    km = (N - k) & (N - 1);
-            const float2 i0 = load_cpx(g0, k), i0m = load_cpx(g0, km);
-            const float2 i1 = load_cpx(g1, k), i1m = load_cpx(g1, km);
-            const float2 h0 = make_float2(i0.x + i0m.x, i0.y - i0m.y); // 2 H_0[k]
-            const float2 h1 = make_float2(i1.x + i1m.x, i1.y - i1m.y); // 2 H_1[k]
-            const float2 d  = csub(h0, h1), sm = cadd(h0, h1);
-            const float2 r  = cadd(cmul(make_float2(v[i].x, -v[i].y), d), cmul(zm[i], sm));
-            v[i]            = make_float2(0.25f * r.x, 0.25f * r.y);
-        }
-    }
-    fft.template run<+1>(v, wl, t);
-    // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
-    // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+    // W_N^{(o + t) q} conj, times the intra scale: the output twiddle's lane part
     const int omask = ((o % N) + N) % N;
     float2    s     = cmul(lw, twm[N + ((omask * q) & (N - 1))]);
     s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
+    float2* wl      = lds + q * FFT::SLOTS;
+
+    for ( int it = 0; it < nitems; it++ ) {
+        const XcorrItem item = items ? items[it] : one;
+        const float2*   ta   = static_cast<const float2*>(item.a);
+        float2*         to   = static_cast<float2*>(item.out);
+        // the previous template's reduction (or the image reads) are done with LDS
+        __syncthreads( );
+        {
+            const float2* ac = ta + col * lda;
+            const float2* a1 = ta + (long)last_line * lda;
+            for ( int n = threadIdx.x; n < item.L; n += NT )
+                s_tpl[n] = ends ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
+        }
+        __syncthreads( );
+        // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
+        float2 v[E];
 #pragma unroll
-    for ( int i = 0; i < E; i++ ) {
-        float2 c = twm[N + ((T * i * q) & (N - 1))];
-        c.y      = -c.y;
-        wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
-    }
-    __syncthreads( );
-    for ( int j = threadIdx.x; j < M; j += NT ) {
-        float2    acc = make_float2(0.f, 0.f);
-        const int sj  = FFT::Layout::slot(j);
-#pragma unroll
-        for ( int w = 0; w < Q; w++ )
-            acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
+        for ( int i = 0; i < E; i++ ) {
+            const int    n = t + T * i;
+            const float2 x = n < item.L ? s_tpl[n] : make_float2(0.f, 0.f);
+            v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
+        }
+        fft.template run<-1>(v, wl, t);
         if ( ! ends ) {
-            out[col * ldo + j * o_es] = acc;
+#pragma unroll
+            for ( int i = 0; i < E; i++ )
+                v[i] = cmulc(g[i], v[i]); // img * conj(X)
         }
         else {
-            out[j * o_es]                         = make_float2(acc.x, 0.f);
-            out[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
+            // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
+            // k'' = (P - k') mod P when q = 0, P - 1 - k' otherwise
+#pragma unroll
+            for ( int i = 0; i < E; i++ )
+                wl[FFT::Layout::slot(t + T * i)] = v[i];
+            __syncthreads( );
+            const int qp = (Q - q) % Q;
+            float2    zm[E];
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                const int kp  = t + T * i;
+                const int kpp = q == 0 ? (P - kp) & (P - 1) : P - 1 - kp;
+                zm[i]         = lds[qp * FFT::SLOTS + FFT::Layout::slot(kpp)];
+            }
+            __syncthreads( );
+            const TImg* g0 = img;
+            const TImg* g1 = img + (long)last_line * ldg;
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                const int    k  = Q * (t + T * i) + q;
+                const int    km = (N - k) & (N - 1);
+                const float2 i0 = load_cpx(g0, k), i0m = load_cpx(g0, km);
+                const float2 i1 = load_cpx(g1, k), i1m = load_cpx(g1, km);
+                const float2 h0 = make_float2(i0.x + i0m.x, i0.y - i0m.y); // 2 H_0[k]
+                const float2 h1 = make_float2(i1.x + i1m.x, i1.y - i1m.y); // 2 H_1[k]
+                const float2 d  = csub(h0, h1), sm = cadd(h0, h1);
+                const float2 r  = cadd(cmul(make_float2(v[i].x, -v[i].y), d), cmul(zm[i], sm));
+                v[i]            = make_float2(0.25f * r.x, 0.25f * r.y);
+            }
+        }
+        fft.template run<+1>(v, wl, t);
+        // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
+        // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            float2 c = twm[N + ((T * i * q) & (N - 1))];
+            c.y      = -c.y;
+            wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
+        }
+        __syncthreads( );
+        for ( int j = threadIdx.x; j < M; j += NT ) {
+            float2    acc = make_float2(0.f, 0.f);
+            const int sj  = FFT::Layout::slot(j);
+#pragma unroll
+            for ( int w = 0; w < Q; w++ )
+                acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
+            if ( ! ends ) {
+                to[col * ldo + j * o_es] = acc;
+            }
+            else {
+                to[j * o_es]                         = make_float2(acc.x, 0.f);
+                to[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
+            }
         }
     }
 }

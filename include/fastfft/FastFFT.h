@@ -226,6 +226,26 @@ class FourierTransformer {
                       "FwdImageInvFFTArgMax");
     }
 
+    // Extension: register n templates (device arrays of sizes_xy[2i] x sizes_xy[2i+1],
+    // FFTW padded, no larger than the forward input size) with their output buffers
+    // and optional device peak / value arrays for FwdImageInvFFTBatch.
+    void SetBatch(int n, const InputType* const* templates, const int* sizes_xy, InputType* const* outputs,
+                  int* peaks_xy = nullptr, float* values = nullptr) {
+        detail::check(fastfft_set_batch(plan_, n, reinterpret_cast<const void* const*>(templates), sizes_xy,
+                                        reinterpret_cast<void* const*>(outputs), peaks_xy, values),
+                      "SetBatch");
+    }
+
+    // Extension: every correlation of the batch against image_to_search, asynchronously.
+    template <class PreOpType = std::nullptr_t, class IntraOpType = std::nullptr_t, class PostOpType = std::nullptr_t>
+    void FwdImageInvFFTBatch(OtherImageType* image_to_search, PreOpType pre_op = nullptr, IntraOpType intra_op = nullptr,
+                             PostOpType post_op = nullptr) {
+        static_assert(HasIntraOpFunctor<IntraOpType>, "FwdImageInvFFT needs an intra op functor (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
+        detail::check(fastfft_fwd_image_inv_batch(plan_, image_to_search, detail::to_op(pre_op), detail::to_op(intra_op),
+                                                  detail::to_op(post_op)),
+                      "FwdImageInvFFTBatch");
+    }
+
     // Extension: argmax of a real [ny][pitch] device image, (x, y) and value.
     void ArgMax(const InputType* img, int nx, int ny, int pitch, int& peak_x, int& peak_y, float& value) {
         int pk[2];

@@ -119,6 +119,25 @@ fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const v
                                             fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op, int* peak_xy,
                                             float* value);
 
+/* Extension (batched correlation): registers n templates for
+ * fastfft_fwd_image_inv_batch.  Template i is a real sizes_xy[2i] x sizes_xy[2i+1]
+ * device array, FFTW padded to its own pitch 2*(x/2+1), no larger than the
+ * plan's forward input size; it is correlated against the image with the
+ * plan's padded size, output window and element types, and its window goes to
+ * outputs[i] (the plan's inverse output layout).  peaks_xy (device int[2n]) and
+ * values (device float[n]) receive each output's argmax, or are both NULL.
+ * Synchronous set-up; the arrays are copied, the device buffers must outlive
+ * the batch.  Replaces any previous batch; a new plan or window clears it. */
+fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* templates, const int* sizes_xy,
+                                 void* const* outputs, int* peaks_xy, float* values);
+
+/* Extension (batched correlation): runs every correlation of the batch set by
+ * fastfft_set_batch against image_to_search, asynchronously on the plan's
+ * stream; free of allocations after the first call, so it can be captured in a
+ * HIP graph.  The ops apply to every correlation as in FwdImageInvFFT. */
+fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_to_search, fastfft_op pre_op,
+                                           fastfft_op intra_op, fastfft_op post_op);
+
 /* Wait() (FastFFT.h:124-126): synchronise the plan's stream. */
 fastfft_status fastfft_wait(fastfft_plan plan);
 

@@ -1,0 +1,112 @@
+"""Batched correlation (many templates of uneven sizes against one spectrum) through the C ABI.
+
+Each output of a batch must agree with the single-template FwdImageInvFFT of the same template (the
+batched kernels use the sub-transform sizes of the largest template, so agreement is to rounding), each
+peak must equal the argmax of its output, and small batches are also checked against the FP64 oracle.  The batch is replayed from a HIP graph with eager work in between.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from . import gpu_helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-6
+
+
+def _setup(sizes, n, nout, origin, seed=0):
+    from fastfft_amd import workloads as W
+
+    torch, d = H.torch_dev()
+    rng = np.random.default_rng(seed)
+    mx = max(s[0] for s in sizes), max(s[1] for s in sizes)
+    ft = H.make_plan((*mx, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((w, n[1])) + 1j * rng.standard_normal((w, n[1]))).astype(np.complex64)
+    img = H.to_dev(H.complex_to_f32(spec))
+    tmpls, hosts, outs = [], [], []
+    for i, (x, y) in enumerate(sizes):
+        t = W.masked_template(x, 0.45 * min(x, y), seed=seed + 100 + i)[:y, :x] if x == y else \
+            rng.standard_normal((y, x)).astype(np.float32)
+        hosts.append(t)
+        tmpls.append(H.to_dev(H.padded_real(t, x).reshape(-1)))
+        outs.append(H.dev_zeros(ft.ReturnInvOutputMemorySize()))
+    peaks = torch.full((2 * len(sizes),), -1, dtype=torch.int32, device=d)
+    vals = torch.zeros(len(sizes), dtype=torch.float32, device=d)
+    ft.SetBatch(tmpls, sizes, outs, peaks, vals)
+    return torch, ft, spec, img, tmpls, hosts, outs, peaks, vals
+
+
+BATCHES = [([(128, 128), (256, 256), (384, 384), (128, 128), (384, 384)], (4096, 4096), (512, 512), (-256, -256)),
+           ([(40, 30), (64, 64), (17, 50)], (256, 128), (64, 32), (-32, -16)),
+           ([(100, 60), (100, 60)], (256, 128), (100, 60), (-50, -30))]
+
+
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "per_template"])
+@pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
+def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
+    """Batched kernels (chunk size from the tuner, or 2 to force several chunks) and the per-template
+    launches all agree with single-template correlations and give the argmax of their own outputs."""
+    if mode == "chunk2":
+        monkeypatch.setenv("FASTFFT_BATCH_CHUNK", "2")
+    if mode == "per_template":
+        monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
+    torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
+    ft.FwdImageInvFFTBatch(img)
+    pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
+    pitch = H.F.padded_pitch(nout[0])
+    for i, (x, y) in enumerate(sizes):
+        single = H.make_plan((x, y, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+        a = H.dev_zeros(max(single.ReturnInputMemorySize(), single.ReturnInvOutputMemorySize()))
+        a[: tmpls[i].numel()] = tmpls[i]
+        o = H.dev_zeros(single.ReturnInvOutputMemorySize())
+        single.FwdImageInvFFT(a, img, o)
+        got = H.real_window(H.to_host(outs[i]), *nout)
+        assert O.rel_l2(got, H.real_window(H.to_host(o), *nout)) < TOL, i
+        rx, ry, rv = single.ArgMax(outs[i], nout[0], nout[1], pitch)  # argmax of the batch's own output
+        assert tuple(pk[i]) == (rx, ry) and vv[i] == np.float32(rv), i
+        if n[0] * n[1] <= 256 * 128:
+            want = O.xcorr_2d(H.padded_real(hosts[i], x), x, spec, n[0], n[1], nout[0], nout[1], *origin)
+            assert O.rel_l2(got, want) < TOL, i
+
+
+def test_batch_graph_replays():
+    sizes, n, nout, origin = [(40, 30), (64, 64), (17, 50)], (256, 128), (64, 32), (-32, -16)
+    H.log("batch in a HIP graph, replayed with eager work in between")
+    torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=9)
+    side = torch.cuda.Stream()
+    ft.SetStream(side.cuda_stream)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        ft.FwdImageInvFFTBatch(img)
+    torch.cuda.synchronize()
+    want = [H.to_host(o).copy() for o in outs]
+    want_pk, want_v = H.to_host(peaks).copy(), H.to_host(vals).copy()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        ft.FwdImageInvFFTBatch(img)
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        peaks.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i, o in enumerate(outs):
+            assert np.array_equal(H.to_host(o), want[i]), (rep, i)
+        assert np.array_equal(H.to_host(peaks), want_pk) and np.array_equal(H.to_host(vals), want_v), rep
+        ft.FwdImageInvFFTBatch(img)
+        torch.cuda.synchronize()
+
+
+def test_batch_rejects_oversized_template():
+    torch, d = H.torch_dev()
+    ft = H.make_plan((64, 64, 1), (256, 256, 1), (256, 256, 1), (64, 64, 1))
+    t = H.dev_zeros(H.F.padded_pitch(128) * 128)
+    o = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    with pytest.raises(H.F.FastFFTError) as e:
+        ft.SetBatch([t], [(128, 128)], [o])
+    assert e.value.status == H.F.INVALID_ARGUMENT
