@@ -458,7 +458,7 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
         chosen     = 1;
         for ( size_t ci = 0; e == hipSuccess && ci < cand.size( ); ci++ ) {
             p->batch_chunk = cand[ci];
-            if ( auto rc = fill_batch_items(p, st, cand[ci]) ) {
+            if ( fill_batch_items(p, st, cand[ci]) != FASTFFT_SUCCESS ) {
                 e = hipErrorUnknown;
                 break;
             }
@@ -726,9 +726,10 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
             continue;
         // the plan with this template size as its forward input: same padded size,
         // window and types, so the workspace of the plan covers it
-        PlanState ps = plan->st;
-        ps.fwd_in    = Dims{x, y, 1, 0};
-        PlanError pe = validate(ps);
+        PlanState ps       = plan->st;
+        ps.fwd_in          = Dims{x, y, 1, 0};
+        ps.input_memory    = padded_memory_size(ps.fwd_in); // also sets fwd_in.w (the row pitch)
+        PlanError pe       = validate(ps);
         std::vector<Stage> st;
         if ( ! pe.code )
             pe = build_stages(ps, "xcorr", st);

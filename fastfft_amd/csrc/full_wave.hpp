@@ -102,3 +102,80 @@ __global__ void __launch_bounds__(64 * Q)
 }
 
 } // namespace fastfft_amd
+
+namespace fastfft_amd {
+
+// Full-length inverse line transforms (M = N outputs from 0): the C2C inverse
+// passes and the C2R x-pass of an InvFFT without size change (the reference's
+// block_fft_kernel_C2C_NONE inverse and C2R_NONE_XY, FastFFT.cu:1369-1403,
+// :1641-1660).  With N = P*Q, decimation in time:
+//   U_q[m]      = IFFT_P( X[Q k' + q] )[m]                       (wave q)
+//   y[m + P s]  = sum_q W_Q^{-s q} ( W_N^{-m q} U_q[m] )         (radix-Q pass in LDS)
+// HERM: the input line holds X[0 .. N/2] of a Hermitian sequence (X[N-k] =
+// conj X[k] above) and only the real part of y is stored (REAL_OUT).
+template <int P, int Q, typename TIn, bool HERM, typename TOut, bool REAL_OUT>
+__global__ void __launch_bounds__(64 * Q)
+        k_inv_full_wave(const TIn* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int nlines,
+                        float post_scale, const float2* __restrict__ tw) {
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
+    using FFT       = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_twn[NT];
+    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t    = threadIdx.x % T;
+    const int     line = xcd_row(blockIdx.x, nlines);
+    const float2* twm  = multi_twiddles(tw);
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
+    // ---- the input line into LDS, decimated by Q (coalesced reads)
+    const TIn* lp = in + im.base(line);
+    for ( int k = threadIdx.x; k <= (HERM ? H : N - 1); k += NT )
+        lds[dec_slot<P, Q>(k)] = load_cpx(lp, k * im.es);
+    __syncthreads( );
+    FFT fft;
+    fft.init_p(t, s_twp);
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int k = Q * (t + T * i) + q;
+        if constexpr ( HERM ) {
+            const bool lo = k <= H;
+            float2     z  = lds[dec_slot<P, Q>(lo ? k : N - k)];
+            v[i]          = lo ? z : make_float2(z.x, -z.y);
+        }
+        else {
+            v[i] = lds[dec_slot<P, Q>(k)];
+        }
+    }
+    __syncthreads( );
+    float2* wl = lds + q * FFT::SLOTS;
+    fft.template run<+1>(v, wl, t);
+    // conj(W_N^{m q}) U_q[m], m = t + 64 i, into LDS at q P + m
+    const float2 lw = s_twn[t * q];
+    __syncthreads( );
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const float2 w = cmul(lw, twm[N + ((T * i * q) & (N - 1))]);
+        lds[q * P + t + T * i] = cmulc(v[i], w);
+    }
+    __syncthreads( );
+    TOut* op = out + om.base(line);
+    for ( int m = threadIdx.x; m < P; m += NT ) {
+        float2 u[Q];
+#pragma unroll
+        for ( int s = 0; s < Q; s++ )
+            u[s] = lds[s * P + m];
+        SmallDFT<Q, +1>::run(u);
+#pragma unroll
+        for ( int s = 0; s < Q; s++ ) {
+            const long j = (long)(m + P * s) * om.es;
+            if constexpr ( REAL_OUT )
+                store_real(op, j, u[s].x * post_scale);
+            else
+                store_cpx(op, j, make_float2(u[s].x * post_scale, u[s].y * post_scale));
+        }
+    }
+}
+
+} // namespace fastfft_amd

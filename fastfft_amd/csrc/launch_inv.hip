@@ -1,4 +1,5 @@
 // Inverse line stages: C2C y/z passes (output pruned) and the C2R x-pass.
+#include "full_wave.hpp"
 #include "launch_common.hpp"
 #include "rows_wave.hpp"
 
@@ -132,7 +133,54 @@ hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT
     return by_p(std::integral_constant<int, 1024>{ });
 }
 
+// Full-length inverse lines (M = N outputs from 0, 256 <= N <= 8192): one
+// workgroup per line, P = 512 (256 for N = 256) and Q = N / P.
+template <int P, int Q>
+static hipError_t go_full_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const dim3 grid((unsigned)s.nlines), block(64 * Q);
+    const auto im = to_device(s.im), om = to_device(s.om);
+    if ( s.kind == StageKind::inv_c2r ) {
+        const auto* in = static_cast<const float2*>(io.src);
+        if ( io.dst_rt == RealT::f32 )
+            k_inv_full_wave<P, Q, float2, true, float, true>
+                    <<<grid, block, 0, st>>>(in, im, static_cast<float*>(io.dst), om, s.nlines, io.scale, tw);
+        else
+            k_inv_full_wave<P, Q, float2, true, __half, true>
+                    <<<grid, block, 0, st>>>(in, im, static_cast<__half*>(io.dst), om, s.nlines, io.scale, tw);
+    }
+    else if ( io.src_ct == CpxT::c64 ) {
+        k_inv_full_wave<P, Q, float2, false, float2, false><<<grid, block, 0, st>>>(
+                static_cast<const float2*>(io.src), im, static_cast<float2*>(io.dst), om, s.nlines, io.scale, tw);
+    }
+    else {
+        k_inv_full_wave<P, Q, __half2, false, float2, false><<<grid, block, 0, st>>>(
+                static_cast<const __half2*>(io.src), im, static_cast<float2*>(io.dst), om, s.nlines, io.scale, tw);
+    }
+    return hipGetLastError( );
+}
+
+static hipError_t try_full_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( s.M != s.N || s.o % s.N != 0 || io.src_real || (io.src_ct == CpxT::c32h && s.kind == StageKind::inv_c2r) )
+        return hipErrorNotSupported;
+    if ( s.kind == StageKind::inv_c2r ? ! io.dst_real : (io.dst_real || io.dst_ct != CpxT::c64) )
+        return hipErrorNotSupported;
+    switch ( s.N ) {
+        case 256: return go_full_inv<256, 1>(s, io, tw, st);
+        case 512: return go_full_inv<512, 1>(s, io, tw, st);
+        case 1024: return go_full_inv<512, 2>(s, io, tw, st);
+        case 2048: return go_full_inv<512, 4>(s, io, tw, st);
+        case 4096: return go_full_inv<512, 8>(s, io, tw, st);
+        case 8192: return go_full_inv<512, 16>(s, io, tw, st);
+        default: return hipErrorNotSupported;
+    }
+}
+
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) && ! env_int("FASTFFT_NO_FULL_LINES", 0) && ! io.amax ) {
+        hipError_t e = try_full_inv(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     if ( s.kind == StageKind::inv_c2r && io.dst_real && ! io.src_real && io.src_ct == CpxT::c64 &&
          ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = io.dst_rt == RealT::f32 ? try_c2r_wave<float>(s, io, tw, st) : try_c2r_wave<__half>(s, io, tw, st);

@@ -24,6 +24,8 @@
 // same amount of work.  Those two columns are stored with a zero imaginary part.
 #pragma once
 
+#include <type_traits>
+
 #include "rows_wave.hpp"
 
 namespace fastfft_amd {
@@ -130,89 +132,99 @@ __global__ void __launch_bounds__(64 * Q)
     s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
     float2* wl      = lds + q * FFT::SLOTS;
 
-    for ( int it = 0; it < nitems; it++ ) {
-        const XcorrItem item = items ? items[it] : one;
-        const float2*   ta   = static_cast<const float2*>(item.a);
-        float2*         to   = static_cast<float2*>(item.out);
-        // the previous template's reduction (or the image reads) are done with LDS
-        __syncthreads( );
-        {
-            const float2* ac = ta + col * lda;
-            const float2* a1 = ta + (long)last_line * lda;
-            for ( int n = threadIdx.x; n < item.L; n += NT )
-                s_tpl[n] = ends ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
-        }
-        __syncthreads( );
-        // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
-        float2 v[E];
-#pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int    n = t + T * i;
-            const float2 x = n < item.L ? s_tpl[n] : make_float2(0.f, 0.f);
-            v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
-        }
-        fft.template run<-1>(v, wl, t);
-        if ( ! ends ) {
-#pragma unroll
-            for ( int i = 0; i < E; i++ )
-                v[i] = cmulc(g[i], v[i]); // img * conj(X)
-        }
-        else {
-            // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
-            // k'' = (P - k') mod P when q = 0, P - 1 - k' otherwise
-#pragma unroll
-            for ( int i = 0; i < E; i++ )
-                wl[FFT::Layout::slot(t + T * i)] = v[i];
+    // The packed end columns take their own copy of the template loop, so that
+    // neither copy holds the other's values across iterations.
+    auto templates = [&](auto ends_c) {
+        constexpr bool ENDS = decltype(ends_c)::value;
+        for ( int it = 0; it < nitems; it++ ) {
+            const XcorrItem item = items ? items[it] : one;
+            const float2*   ta   = static_cast<const float2*>(item.a);
+            float2*         to   = static_cast<float2*>(item.out);
+            // the previous template's reduction (or the image reads) are done with LDS
             __syncthreads( );
-            const int qp = (Q - q) % Q;
-            float2    zm[E];
-#pragma unroll
-            for ( int i = 0; i < E; i++ ) {
-                const int kp  = t + T * i;
-                const int kpp = q == 0 ? (P - kp) & (P - 1) : P - 1 - kp;
-                zm[i]         = lds[qp * FFT::SLOTS + FFT::Layout::slot(kpp)];
+            {
+                const float2* ac = ta + col * lda;
+                const float2* a1 = ta + (long)last_line * lda;
+                for ( int n = threadIdx.x; n < item.L; n += NT )
+                    s_tpl[n] = ENDS ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
             }
             __syncthreads( );
-            const TImg* g0 = img;
-            const TImg* g1 = img + (long)last_line * ldg;
+            // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
+            float2 v[E];
 #pragma unroll
             for ( int i = 0; i < E; i++ ) {
-                const int    k  = Q * (t + T * i) + q;
-                const int    km = (N - k) & (N - 1);
-                const float2 i0 = load_cpx(g0, k), i0m = load_cpx(g0, km);
-                const float2 i1 = load_cpx(g1, k), i1m = load_cpx(g1, km);
-                const float2 h0 = make_float2(i0.x + i0m.x, i0.y - i0m.y); // 2 H_0[k]
-                const float2 h1 = make_float2(i1.x + i1m.x, i1.y - i1m.y); // 2 H_1[k]
-                const float2 d  = csub(h0, h1), sm = cadd(h0, h1);
-                const float2 r  = cadd(cmul(make_float2(v[i].x, -v[i].y), d), cmul(zm[i], sm));
-                v[i]            = make_float2(0.25f * r.x, 0.25f * r.y);
+                const int    n = t + T * i;
+                const float2 x = n < item.L ? s_tpl[n] : make_float2(0.f, 0.f);
+                v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
             }
-        }
-        fft.template run<+1>(v, wl, t);
-        // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
-        // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+            fft.template run<-1>(v, wl, t);
+            if constexpr ( ! ENDS ) {
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            float2 c = twm[N + ((T * i * q) & (N - 1))];
-            c.y      = -c.y;
-            wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
-        }
-        __syncthreads( );
-        for ( int j = threadIdx.x; j < M; j += NT ) {
-            float2    acc = make_float2(0.f, 0.f);
-            const int sj  = FFT::Layout::slot(j);
-#pragma unroll
-            for ( int w = 0; w < Q; w++ )
-                acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
-            if ( ! ends ) {
-                to[col * ldo + j * o_es] = acc;
+                for ( int i = 0; i < E; i++ )
+                    v[i] = cmulc(g[i], v[i]); // img * conj(X)
             }
             else {
-                to[j * o_es]                         = make_float2(acc.x, 0.f);
-                to[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
+                // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
+                // k'' = (P - k') mod P when q = 0, P - 1 - k' otherwise (the partner
+                // wave's region).  The image values are reloaded for every template;
+                // the opaque pointers and lane index keep these template-invariant
+                // loads and indices from being hoisted out of the loop into registers.
+#pragma unroll
+                for ( int i = 0; i < E; i++ )
+                    wl[FFT::Layout::slot(t + T * i)] = v[i];
+                __syncthreads( );
+                const int   qp = (Q - q) % Q;
+                const TImg* g0 = img;
+                const TImg* g1 = img + (long)last_line * ldg;
+                int         tt = t;
+                asm volatile("" : "+s"(g0), "+s"(g1), "+v"(tt));
+#pragma unroll
+                for ( int i = 0; i < E; i++ ) {
+                    const int    kp  = tt + T * i;
+                    const int    kpp = q == 0 ? (P - kp) & (P - 1) : P - 1 - kp;
+                    const float2 zm  = lds[qp * FFT::SLOTS + FFT::Layout::slot(kpp)];
+                    const int    k   = Q * kp + q;
+                    const int    km  = (N - k) & (N - 1);
+                    const float2 i0 = load_cpx(g0, k), i0m = load_cpx(g0, km);
+                    const float2 i1 = load_cpx(g1, k), i1m = load_cpx(g1, km);
+                    const float2 h0 = make_float2(i0.x + i0m.x, i0.y - i0m.y); // 2 H_0[k]
+                    const float2 h1 = make_float2(i1.x + i1m.x, i1.y - i1m.y); // 2 H_1[k]
+                    const float2 d  = csub(h0, h1), sm = cadd(h0, h1);
+                    const float2 r  = cadd(cmul(make_float2(v[i].x, -v[i].y), d), cmul(zm, sm));
+                    v[i]            = make_float2(0.25f * r.x, 0.25f * r.y);
+                }
+                __syncthreads( );
+            }
+            fft.template run<+1>(v, wl, t);
+            // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
+            // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                float2 c = twm[N + ((T * i * q) & (N - 1))];
+                c.y      = -c.y;
+                wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
+            }
+            __syncthreads( );
+            for ( int j = threadIdx.x; j < M; j += NT ) {
+                float2    acc = make_float2(0.f, 0.f);
+                const int sj  = FFT::Layout::slot(j);
+#pragma unroll
+                for ( int w = 0; w < Q; w++ )
+                    acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
+                if constexpr ( ! ENDS ) {
+                    to[col * ldo + j * o_es] = acc;
+                }
+                else {
+                    to[j * o_es]                         = make_float2(acc.x, 0.f);
+                    to[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
+                }
             }
         }
-    }
+    };
+    if ( ends )
+        templates(std::true_type{ });
+    else
+        templates(std::false_type{ });
 }
 
 } // namespace fastfft_amd

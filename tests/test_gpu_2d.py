@@ -212,9 +212,12 @@ FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (25
              ((256, 256), (64, 32))]
 
 
+@pytest.mark.parametrize("mode", ["default", "generic"])
 @pytest.mark.parametrize("fin,fout", FWD_CASES)
-def test_fwd_random(fin, fout):
-    H.log(f"fwd random {fin} -> {fout}")
+def test_fwd_random(fin, fout, mode, monkeypatch):
+    if mode == "generic":
+        monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    H.log(f"fwd random {fin} -> {fout} mode={mode}")
     img = rand_img(fin[0], fin[1], seed=fin[0] + 7 * fin[1])
     ft = H.make_plan((*fin, 1), (*fout, 1), (*fout, 1), (*fout, 1))
     n = max(ft.ReturnInputMemorySize(), ft.ReturnFwdOutputMemorySize())
@@ -229,9 +232,12 @@ INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256,
              ((1024, 1024), (1000, 1000), (3, -5))]
 
 
+@pytest.mark.parametrize("mode", ["default", "generic"])
 @pytest.mark.parametrize("nin,nout,origin", INV_CASES)
-def test_inv_random(nin, nout, origin):
-    H.log(f"inv random {nin} -> {nout} at {origin}")
+def test_inv_random(nin, nout, origin, mode, monkeypatch):
+    if mode == "generic":
+        monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    H.log(f"inv random {nin} -> {nout} at {origin} mode={mode}")
     rng = np.random.default_rng(nin[0] + nout[1])
     w = nin[0] // 2 + 1
     spec = (rng.standard_normal((w, nin[1])) + 1j * rng.standard_normal((w, nin[1]))).astype(np.complex64)

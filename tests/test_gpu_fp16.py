@@ -2,6 +2,9 @@
 
 Templates and micrographs are stored as FP16, the micrograph spectrum as __half2 (produced by the
 library's own FwdFFT of the FP16 micrograph), the correlation output as FP16; all arithmetic is FP32.
+The transforms are unnormalised, so FP16 storage needs the SCALE functors a caller would use: the
+micrograph FwdFFT is pre-scaled by 1/n (spectrum values of order 1) and the correlation post-scaled by
+1/(n t^2) (peak of order 0.3), both applied identically in the oracle.
 The reference for each case is the FP64 oracle on the same FP16-rounded inputs, with the spectrum the
 oracle computes from the FP16 micrograph, so the measured error contains the FP16 storage of the
 spectrum and of the output.  Peaks must be identical to the oracle's and to the planted lag.
@@ -36,14 +39,14 @@ def _case(n, t, m, radius, n_templates, seed):
     # micrograph spectrum on the GPU as __half2
     mft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1), input_type="f16", image_type="c32h")
     mbuf = H.to_dev(W.to_padded(mic16, dtype=np.float16).reshape(-1))
-    mft.FwdFFT(mbuf)
+    mft.FwdFFT(mbuf, pre_op=H.F.Scale(1.0 / n))
     return torch, d, tmpls, lags, mic16, mbuf
 
 
 def _spec_f64(mic16, n):
     from fastfft_amd import workloads as W
 
-    return O.fwd_2d(W.to_padded(mic16.astype(np.float32)), n, n, n)
+    return O.fwd_2d(W.to_padded(mic16.astype(np.float32)), n, n, n, pre_scale=1.0 / n)
 
 
 @pytest.mark.parametrize("n,t,m", [(1024, 128, 256), (4096, 512, 512)])
@@ -60,9 +63,11 @@ def test_fp16_correlation_vs_oracle(n, t, m):
     out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), "f16")
     peak = torch.zeros(2, dtype=torch.int32, device=d)
     val = torch.zeros(1, dtype=torch.float32, device=d)
-    ft.FwdImageInvFFTArgMax(a, mbuf, out, peak, val)
+    post = 1.0 / (n * t * t)
+    ft.FwdImageInvFFTArgMax(a, mbuf, out, peak, val, post_op=H.F.Scale(post))
     got = H.real_window(H.to_host(out).astype(np.float32), m, m)
-    want = O.xcorr_2d(W.to_padded(tmpls[0].astype(np.float32)), t, _spec_f64(mic16, n), n, n, m, m, -m // 2, -m // 2)
+    want = O.xcorr_2d(W.to_padded(tmpls[0].astype(np.float32)), t, _spec_f64(mic16, n), n, n, m, m, -m // 2, -m // 2,
+                      post_scale=post)
     err = O.rel_l2(got, want)
     print(f"fp16 {n}/{t}/{m}: rel_l2={err:.3e}")
     assert err < TOL_F16
@@ -86,11 +91,13 @@ def test_fp16_batch_config5_shape():
     peaks = torch.zeros(2 * len(tmpls), dtype=torch.int32, device=d)
     vals = torch.zeros(len(tmpls), dtype=torch.float32, device=d)
     ft.SetBatch(tb, [(t, t)] * len(tmpls), outs, peaks, vals)
-    ft.FwdImageInvFFTBatch(mbuf)
+    post = 1.0 / (n * t * t)
+    ft.FwdImageInvFFTBatch(mbuf, post_op=H.F.Scale(post))
     pk = H.to_host(peaks).reshape(-1, 2)
     for i in range(len(tmpls)):
         assert (pk[i][0] - m // 2, pk[i][1] - m // 2) == tuple(lags[i]), i
     # template 0 against the oracle
     got = H.real_window(H.to_host(outs[0]).astype(np.float32), m, m)
-    want = O.xcorr_2d(W.to_padded(tmpls[0].astype(np.float32)), t, _spec_f64(mic16, n), n, n, m, m, -m // 2, -m // 2)
+    want = O.xcorr_2d(W.to_padded(tmpls[0].astype(np.float32)), t, _spec_f64(mic16, n), n, n, m, m, -m // 2, -m // 2,
+                      post_scale=post)
     assert O.rel_l2(got, want) < TOL_F16
