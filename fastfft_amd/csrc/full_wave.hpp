@@ -64,8 +64,8 @@ __global__ void __launch_bounds__(64 * Q)
     }
     __syncthreads( );
     FFT fft;
-    fft.init_p(t, s_twp);
-    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    fft.init_pt(t, s_twp);
+    const float2 lw = lane_twiddle(s_twn, t, q); // W_N^{t q}
     float2       v[E];
 #pragma unroll
     for ( int i = 0; i < E; i++ )
@@ -128,13 +128,18 @@ __global__ void __launch_bounds__(64 * Q)
     const int     line = xcd_row(blockIdx.x, nlines);
     const float2* twm  = multi_twiddles(tw);
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    // ---- the input line into LDS, decimated by Q (coalesced reads)
-    const TIn* lp = in + im.base(line);
-    for ( int k = threadIdx.x; k <= (HERM ? H : N - 1); k += NT )
-        lds[dec_slot<P, Q>(k)] = load_cpx(lp, k * im.es);
+    // ---- the input line into LDS, decimated by Q, two values per lane (the
+    // decimated layout is conflict-free for pairs)
+    const TIn* lp   = in + im.base(line);
+    constexpr int KN = HERM ? H + 1 : N;
+    for ( int m = threadIdx.x; 2 * m < KN; m += NT ) {
+        lds[dec_slot<P, Q>(2 * m)] = load_cpx(lp, 2 * m * im.es);
+        if ( 2 * m + 1 < KN )
+            lds[dec_slot<P, Q>(2 * m + 1)] = load_cpx(lp, (2 * m + 1) * im.es);
+    }
     __syncthreads( );
     FFT fft;
-    fft.init_p(t, s_twp);
+    fft.init_pt(t, s_twp);
     float2 v[E];
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
@@ -152,7 +157,7 @@ __global__ void __launch_bounds__(64 * Q)
     float2* wl = lds + q * FFT::SLOTS;
     fft.template run<+1>(v, wl, t);
     // conj(W_N^{m q}) U_q[m], m = t + 64 i, into LDS at q P + m
-    const float2 lw = s_twn[t * q];
+    const float2 lw = lane_twiddle(s_twn, t, q);
     __syncthreads( );
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {

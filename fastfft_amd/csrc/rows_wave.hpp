@@ -41,24 +41,31 @@ __device__ __forceinline__ int xcd_row(int b, int nrows) {
 }
 
 // LDS slot of element k = Q k' + q of a row: the Q decimated sequences one after
-// another, each XOR-swizzled inside blocks of 64 slots, so that wave q's reads
+// another, each XOR-swizzled inside blocks of 32 slots, so that wave q's reads
 // or writes of consecutive k' and the row-order accesses of two consecutive k
-// per lane are free of bank conflicts.
+// per lane are free of bank conflicts (64 banks of 4 bytes serve a 64-bit
+// access 32 lanes at a time; the 32 lanes of such a group then hold 64/Q
+// values of k' for each of the Q/2 even-odd pairs of q, which the swizzle
+// moves to distinct groups of 64/Q slots).
 template <int P, int Q>
 __device__ __forceinline__ int dec_slot(int k) {
     const int q = k % Q, kp = k / Q;
-    return q * P + (kp ^ (((q >> 1) * (128 / Q)) & 63));
+    return q * P + (kp ^ (((q >> 1) * (64 / Q)) & 31));
 }
 
-// LDS copies of W_P^k (k < P) and W_N^k (k < 64 Q) from the multi-resolution table.
+// LDS tables of the twiddles a workgroup of Q waves needs, laid out so that
+// every wave reads consecutive slots: s_twp = the Stockham pass table of
+// WaveFFT<P> (init_pt), s_twn[64 q + t] = W_N^{t q} (lane_twiddle).
 template <int P, int Q>
 __device__ __forceinline__ void stage_row_twiddles(float2* s_twp, float2* s_twn, const float2* __restrict__ twm) {
     constexpr int N = P * Q, NT = 64 * Q;
-    for ( int k = threadIdx.x; k < P; k += NT )
-        s_twp[k] = twm[P + k];
+    WaveFFT<P>::fill_pass_table(s_twp, twm, threadIdx.x, NT);
     for ( int k = threadIdx.x; k < NT; k += NT )
-        s_twn[k] = twm[N + k];
+        s_twn[k] = twm[N + (k & 63) * (k >> 6)];
 }
+
+// W_N^{t q} of lane t of wave q from the table of stage_row_twiddles
+__device__ __forceinline__ float2 lane_twiddle(const float2* s_twn, int t, int q) { return s_twn[64 * q + t]; }
 
 // items == nullptr: one row per workgroup of in / out; else row blockIdx.x of
 // template items[blockIdx.y] (rows past its count return at once).
@@ -92,8 +99,8 @@ __global__ void __launch_bounds__(64 * Q)
         s_row[n] = load_real(rp, n * im.es) * pre_scale;
     __syncthreads( );
     FFT fft;
-    fft.init_p(t, s_twp);
-    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    fft.init_pt(t, s_twp);
+    const float2 lw = lane_twiddle(s_twn, t, q); // W_N^{t q}
     float2       v[E];
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
@@ -178,12 +185,28 @@ __global__ void __launch_bounds__(64 * Q)
     const float2* cp     = in + im.base(row);
     const bool    staged = im.es == 1;
     if ( staged ) {
-        for ( int k = threadIdx.x; k <= H; k += NT )
-            lds[dec_slot<P, Q>(k)] = cp[k];
+        // two values per lane (the decimated layout is conflict-free for pairs),
+        // 16-byte loads when the row start is 16-byte aligned
+        const bool a16 = (reinterpret_cast<unsigned long>(cp) & 15) == 0;
+        for ( int m = threadIdx.x; 2 * m <= H; m += NT ) {
+            float2 u0, u1;
+            if ( a16 && 2 * m + 1 <= H ) {
+                const float4 f = reinterpret_cast<const float4*>(cp)[m];
+                u0             = make_float2(f.x, f.y);
+                u1             = make_float2(f.z, f.w);
+            }
+            else {
+                u0 = cp[2 * m];
+                u1 = 2 * m + 1 <= H ? cp[2 * m + 1] : make_float2(0.f, 0.f);
+            }
+            lds[dec_slot<P, Q>(2 * m)] = u0;
+            if ( 2 * m + 1 <= H )
+                lds[dec_slot<P, Q>(2 * m + 1)] = u1;
+        }
     }
     __syncthreads( );
     FFT fft;
-    fft.init_p(t, s_twp);
+    fft.init_pt(t, s_twp);
     // X[k] for k <= N/2, conj(X[N - k]) above
     float2 v[E];
 #pragma unroll
@@ -203,7 +226,7 @@ __global__ void __launch_bounds__(64 * Q)
     // Re(U_q[(o + j) mod P] conj(W_N^{(o + j) q})), W_N^{(o+t+64i) q} = W_N^{o q} W_N^{t q} W_N^{64 i q}
     float*       red   = reinterpret_cast<float*>(wl);
     const int    omask = ((o % N) + N) % N;
-    const float2 s     = cmul(s_twn[t * q], twm[N + ((omask * q) & (N - 1))]);
+    const float2 s     = cmul(lane_twiddle(s_twn, t, q), twm[N + ((omask * q) & (N - 1))]);
 #pragma unroll
     for ( int i = 0; i < E; i++ ) {
         const float2 w = cmul(s, twm[N + ((T * i * q) & (N - 1))]);

@@ -183,6 +183,57 @@ struct WaveFFT {
     // Stockham twiddles from a table of W_P^k, k < P (for example a copy in LDS)
     __device__ __forceinline__ void init_p(int t, const float2* tab) { init_pass<1, 0, P>(t, tab); }
 
+    // ---- pass table: the Stockham twiddles of passes 1.. laid out pass by pass,
+    // [r-1][j] within a pass (j = the lane's index mod Ns), so that the reads of
+    // one (pass, r) by the 64 lanes are consecutive LDS slots (no bank conflicts).
+    static constexpr int pass_table_size( ) {
+        int n = 0;
+        for ( int ns = 1; ns < P; ) {
+            const int R = (P / ns) < E ? (P / ns) : E;
+            if ( ns > 1 )
+                n += ns * (R - 1);
+            ns *= R;
+        }
+        return n;
+    }
+    static constexpr int PT = pass_table_size( );
+    static_assert(PT <= P, "the pass table fits a P-entry buffer");
+
+    // Fills the pass table pt[0 .. PT) from the multi-resolution twiddle table
+    // (twm[n + m] = W_n^m), threads `first`, `first + stride`, ... of a workgroup.
+    template <int Ns = 1, int OFF = 0>
+    __device__ __forceinline__ static void fill_pass_table(float2* pt, const float2* __restrict__ twm, int first, int stride) {
+        if constexpr ( Ns < P ) {
+            constexpr int R = (P / Ns) < E ? (P / Ns) : E;
+            if constexpr ( Ns > 1 ) {
+                for ( int k = first; k < Ns * (R - 1); k += stride )
+                    pt[OFF + k] = twm[Ns * R + (k & (Ns - 1)) * (1 + k / Ns)];
+            }
+            fill_pass_table<Ns * R, OFF + (Ns > 1 ? Ns * (R - 1) : 0)>(pt, twm, first, stride);
+        }
+    }
+
+    template <int Ns, int PI, int OFF>
+    __device__ __forceinline__ void init_pass_pt(int t, const float2* pt) {
+        if constexpr ( Ns < P ) {
+            constexpr int R = (P / Ns) < E ? (P / Ns) : E;
+            constexpr int B = E / R;
+            if constexpr ( Ns > 1 ) {
+#pragma unroll
+                for ( int m = 0; m < B; m++ ) {
+                    const int j = (t + T * m) & (Ns - 1);
+#pragma unroll
+                    for ( int r = 1; r < R; r++ )
+                        w[PI - 1][m * (R - 1) + r - 1] = pt[OFF + (r - 1) * Ns + j];
+                }
+            }
+            init_pass_pt<Ns * R, PI + 1, OFF + (Ns > 1 ? Ns * (R - 1) : 0)>(t, pt);
+        }
+    }
+
+    // Stockham twiddles from the pass table (fill_pass_table), e.g. in LDS
+    __device__ __forceinline__ void init_pt(int t, const float2* pt) { init_pass_pt<1, 0, 0>(t, pt); }
+
     // One Stockham pass applied to NB independent transforms held by the wave;
     // each transform exchanges through its own LDS region of SLOTS values, so the
     // NB exchanges of a pass share one LDS round trip.

@@ -86,8 +86,8 @@ __global__ void __launch_bounds__(64 * Q)
     static_assert(E % PER == 0, "whole 16-byte loads per thread");
     static_assert(RS >= 0 && RS < E, "register rotation out of range");
     __shared__ float2 lds[Q * FFT::SLOTS];
-    __shared__ float2 s_twp[P];  // W_P^k
-    __shared__ float2 s_twn[NW]; // W_N^k, k < 64 Q (covers t q)
+    __shared__ float2 s_twp[P];  // Stockham pass table of WaveFFT<P>
+    __shared__ float2 s_twn[NW]; // W_N^{t q} by wave (lane_twiddle)
     __shared__ float2 s_tpl[P];  // template column, first L values
     const int      q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int      t    = threadIdx.x % T;
@@ -100,10 +100,7 @@ __global__ void __launch_bounds__(64 * Q)
         nitems = 1;
 
     // ---- stage: twiddle tables, image column
-    for ( int k = threadIdx.x; k < P; k += NT )
-        s_twp[k] = twm[P + k];
-    for ( int k = threadIdx.x; k < NW; k += NT )
-        s_twn[k] = twm[N + k];
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
     if ( ! ends ) {
         const TImg* gc = img + col * ldg;
 #pragma unroll
@@ -118,24 +115,38 @@ __global__ void __launch_bounds__(64 * Q)
     }
     __syncthreads( );
     FFT fft;
-    fft.init_p(t, s_twp);
-    const float2 lw = s_twn[t * q]; // W_N^{t q}
+    fft.init_pt(t, s_twp);
+    const float2 lw = lane_twiddle(s_twn, t, q); // W_N^{t q}
+    // The output twiddle of output j = t + 64 i is
+    //   conj(W_N^{(o + j) q}) = conj(W_N^{o q}) conj(W_N^{(t + 64 i) q}),
+    // the conjugate of the input twiddle times a wave-uniform factor; with the
+    // intra scale, sigma = conj(W_N^{o q}) * s is folded into the image values
+    // (the inverse transform is linear), so that input and output each take one
+    // complex product per value.
+    const int    omask = ((o % N) + N) % N;
+    const float2 wo    = twm[N + ((omask * q) & (N - 1))];
+    const float2 sigma = make_float2(wo.x * intra_scale, -wo.y * intra_scale);
     float2       g[E];
     if ( ! ends ) {
 #pragma unroll
         for ( int i = 0; i < E; i++ )
-            g[i] = lds[dec_slot<P, Q>(Q * (t + T * i) + q)];
+            g[i] = cmul(lds[dec_slot<P, Q>(Q * (t + T * i) + q)], sigma);
     }
-    // W_N^{(o + t) q} conj, times the intra scale: the output twiddle's lane part
-    const int omask = ((o % N) + N) % N;
-    float2    s     = cmul(lw, twm[N + ((omask * q) & (N - 1))]);
-    s               = make_float2(s.x * intra_scale, -s.y * intra_scale);
-    float2* wl      = lds + q * FFT::SLOTS;
+    // end columns: conj(W_N^{(o + t) q}) * s, the output twiddle's lane part
+    const float2 s  = cmul(make_float2(lw.x, -lw.y), sigma);
+    float2*      wl = lds + q * FFT::SLOTS;
 
     // The packed end columns take their own copy of the template loop, so that
     // neither copy holds the other's values across iterations.
     auto templates = [&](auto ends_c) {
         constexpr bool ENDS = decltype(ends_c)::value;
+        // W_N^{(t + 64 i) q}, held for all templates on the common path
+        float2 win[ENDS ? 1 : E];
+        if constexpr ( ! ENDS ) {
+#pragma unroll
+            for ( int i = 0; i < E; i++ )
+                win[i] = cmul(twm[N + ((T * i * q) & (N - 1))], lw);
+        }
         for ( int it = 0; it < nitems; it++ ) {
             const XcorrItem item = items ? items[it] : one;
             const float2*   ta   = static_cast<const float2*>(item.a);
@@ -155,13 +166,16 @@ __global__ void __launch_bounds__(64 * Q)
             for ( int i = 0; i < E; i++ ) {
                 const int    n = t + T * i;
                 const float2 x = n < item.L ? s_tpl[n] : make_float2(0.f, 0.f);
-                v[i]           = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
+                if constexpr ( ! ENDS )
+                    v[i] = cmul(x, win[i]);
+                else
+                    v[i] = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
             }
             fft.template run<-1>(v, wl, t);
             if constexpr ( ! ENDS ) {
 #pragma unroll
                 for ( int i = 0; i < E; i++ )
-                    v[i] = cmulc(g[i], v[i]); // img * conj(X)
+                    v[i] = cmulc(g[i], v[i]); // img * sigma * conj(X)
             }
             else {
                 // Z[-k] for k = Q k' + q: -k = Q k'' + q' with q' = (Q - q) mod Q and
@@ -196,13 +210,18 @@ __global__ void __launch_bounds__(64 * Q)
                 __syncthreads( );
             }
             fft.template run<+1>(v, wl, t);
-            // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i:
-            // conj(W_N^{o q} W_N^{t q} W_N^{64 i q}) * intra_scale
+            // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i
+            // (sigma already applied on the common path)
 #pragma unroll
             for ( int i = 0; i < E; i++ ) {
-                float2 c = twm[N + ((T * i * q) & (N - 1))];
-                c.y      = -c.y;
-                wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
+                if constexpr ( ! ENDS ) {
+                    wl[FFT::Layout::slot(t + T * i)] = cmulc(v[(i + RS) % E], win[i]);
+                }
+                else {
+                    float2 c = twm[N + ((T * i * q) & (N - 1))];
+                    c.y      = -c.y;
+                    wl[FFT::Layout::slot(t + T * i)] = cmul(cmul(v[(i + RS) % E], c), s);
+                }
             }
             __syncthreads( );
             for ( int j = threadIdx.x; j < M; j += NT ) {
