@@ -35,7 +35,7 @@ def run_fwd(ft, host_padded, nfloats):
     return buf
 
 
-# ------------------------------------------------------------------ round trip
+# ------------------------------------------------------------------ round trip 64^2 into 512^2
 def test_round_trip_64_into_512():
     H.log("round trip 64x64 -> 512x512 fwd, 512 inv")
     img = rand_img(64, 64, seed=1)
@@ -339,7 +339,7 @@ def test_xcorr_row_kernels(tin, n, nout, origin, mode, monkeypatch):
 def test_xcorr_4096_384_trim_512_centred():
     from fastfft_amd import workloads as W
 
-    H.log("config-2 correlation 4096^2 x 384^2 -> 512^2 centred")
+    H.log("correlation 4096^2 x 384^2 -> 512^2 centred")
     mic, tmpl, lags = W.xcorr_config()
     # micrograph spectrum on the GPU (no size change), checked against the oracle
     mft = H.make_plan((4096, 4096, 1), (4096, 4096, 1), (4096, 4096, 1), (4096, 4096, 1))
@@ -358,7 +358,7 @@ def test_xcorr_4096_384_trim_512_centred():
     got = H.real_window(H.to_host(out), 512, 512)
     want = O.xcorr_2d(W.to_padded(tmpl), 384, mspec, 4096, 4096, 512, 512, -256, -256)
     err = O.rel_l2(got, want)
-    print(f"config-2 rel_l2={err:.3e} max_abs={O.max_abs(got, want):.3e} max={np.abs(want).max():.3e}")
+    print(f"4096/384/512 rel_l2={err:.3e} max_abs={O.max_abs(got, want):.3e} max={np.abs(want).max():.3e}")
     assert err < TOL
     gx, gy, _ = O.argmax_2d(got)
     wx, wy, _ = O.argmax_2d(want)

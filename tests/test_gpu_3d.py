@@ -1,7 +1,7 @@
 """3-D parity through the C ABI (Rank = 3 FourierTransformer).
 
 Small shapes are compared element by element with the FP64 oracle.  The full
-config-4 sizes (64^3 -> 256^3 padded forward, 512^3 -> 128^3 trimmed inverse)
+sizes (64^3 -> 256^3 padded forward, 512^3 -> 128^3 trimmed inverse)
 are checked where the oracle would take minutes: against the transforms
 evaluated directly in FP64 at sampled frequencies / output points (the DFT
 sums are separable, so each sample costs one pass over the data), and the
@@ -98,9 +98,9 @@ def test_xcorr_3d_vs_oracle(tin, n, nout, origin):
     assert O.rel_l2(got, want) < TOL
 
 
-# ------------------------------------------------------------------ config 4 at full size
-def test_config4_fwd_64_to_256_sampled():
-    H.log("config-4 fwd 64^3 -> 256^3, sampled DFT")
+# ------------------------------------------------------------------ full sizes
+def test_full_fwd_64_to_256_sampled():
+    H.log("full fwd 64^3 -> 256^3, sampled DFT")
     n = 256
     v = _vol((64, 64, 64), seed=4)
     got = _fwd_gpu(v, (n, n, n))
@@ -119,10 +119,10 @@ def test_config4_fwd_64_to_256_sampled():
     assert O.rel_l2(got[kz, kx, ky], want) < TOL
 
 
-def test_config4_inv_512_to_128_sampled_and_cropped():
+def test_full_inv_512_to_128_sampled_and_cropped():
     from fastfft_amd import workloads as W
 
-    H.log("config-4 inv 512^3 -> 128^3 centred, sampled sums and untrimmed crop")
+    H.log("full inv 512^3 -> 128^3 centred, sampled sums and untrimmed crop")
     n, m, o = 512, 128, -64
     spec = W.hermitian_spectrum_3d(n, seed=7)
     ft = H.make_plan((n, n, n), (n, n, n), (n, n, n), (m, m, m), origin=(o, o, o), rank=3)

@@ -76,13 +76,13 @@ def test_fp16_correlation_vs_oracle(n, t, m):
     assert (wx - m // 2, wy - m // 2) == tuple(lags[0])
 
 
-def test_fp16_batch_config5_shape():
-    """config-5 shape: 4096^2 FP16 micrograph, 512^2 FP16 templates (radius 220), 512^2 window; a batch
+def test_fp16_batch_4096_512_templates():
+    """Full size: 4096^2 FP16 micrograph, 512^2 FP16 templates (radius 220), 512^2 window; a batch
     of 4 templates, each planted once inside the window; peaks equal the planted lags."""
     from fastfft_amd import workloads as W
 
     n, t, m = 4096, 512, 512
-    H.log("fp16 batch, config-5 shape")
+    H.log("fp16 batch 4096^2 x 512^2 -> 512^2")
     torch, d, tmpls, lags, mic16, mbuf = _case(n, t, m, 220.0, 4, seed=5)
     ft = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0), input_type="f16",
                      image_type="c32h")

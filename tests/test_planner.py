@@ -90,7 +90,7 @@ def test_padded_flag_must_match():
         ft.SetInverseFFTPlan(64, 64, 1, 64, 64, 1, False)
 
 
-def test_stage_lists_config2():
+def test_stage_lists_xcorr_4096_384_512():
     ft = plan((384, 384, 1), (4096, 4096, 1), (4096, 4096, 1), (512, 512, 1), origin=(-256, -256, 0))
     d = ft.Describe("xcorr").splitlines()[1:]
     assert [s.split()[0] for s in d] == ["fwd_real", "xcorr", "inv_c2r"]
