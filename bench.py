@@ -192,6 +192,7 @@ def main() -> int:
             "ms_per_corr": round(b_ms, 5),
             "hbm_frac_compulsory": None,  # filled below
             "set_batch_s": round(set_s, 3),
+            "form": bft.Describe("batch").splitlines()[0],
             "peak_lag_template0": [int(bpk_h[0]) - M // 2, int(bpk_h[1]) - M // 2],
             "max_rel_diff_template0_vs_single": float(f"{same:.3e}"),
             "timing": f"HIP graph of one FwdImageInvFFTBatch call ({B} correlations) replayed {breps} times",

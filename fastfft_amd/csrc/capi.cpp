@@ -101,6 +101,21 @@ struct fastfft_plan_s {
     RowBatchItem* batch_r2c   = nullptr;
     XcorrItem*    batch_xc    = nullptr;
     RowBatchItem* batch_c2r   = nullptr;
+    // Sub-frame form of the batch (set_batch decides).  With templates of at most
+    // Lx x Ly, a window of Mx x My lags at origin (ox, oy) and Fx >= Lx + Mx - 1,
+    // Fy >= Ly + My - 1, every kept lag of the N-frame correlation,
+    //   out[x][y] = sum_{u<Lx, v<Ly} t[u][v] g[(ox + x + u) mod N][(oy + y + v) mod N]
+    // (g = unnormalised 2-D inverse of the image spectrum), reads g only inside
+    // the Fx x Fy frame starting at (ox, oy).  Per call the image is reduced once
+    // to that frame (sub_inv: trimmed inverse, scaled by 1/(Fx Fy)) and its
+    // spectrum (sub_fwd); the templates are then correlated on the small frame
+    // (sub_xc, window origin 0) with identical results up to rounding.
+    bool               sub_active = false;
+    int                sub_fx = 0, sub_fy = 0;
+    std::vector<Stage> sub_inv, sub_fwd, sub_xc;
+    void*              sub_dev = nullptr; // frame (real, FFTW padded), its spectrum, workspace a, b
+    float*             sub_s   = nullptr;
+    float2*            sub_h = nullptr, *sub_wa = nullptr, *sub_wb = nullptr;
 };
 
 static int op_index(const char* op) {
@@ -127,11 +142,26 @@ static void free_batch_device(fastfft_plan p) {
     p->batch_chunk = 0;
 }
 
+static void free_subframe(fastfft_plan p) {
+    if ( p->sub_dev )
+        (void)hipFree(p->sub_dev);
+    p->sub_dev    = nullptr;
+    p->sub_s      = nullptr;
+    p->sub_h      = nullptr;
+    p->sub_wa     = nullptr;
+    p->sub_wb     = nullptr;
+    p->sub_active = false;
+    p->sub_inv.clear( );
+    p->sub_fwd.clear( );
+    p->sub_xc.clear( );
+}
+
 static void invalidate(fastfft_plan p) {
     p->st.validated = false;
     p->batch.clear( );
     p->batch_stages.clear( );
     free_batch_device(p);
+    free_subframe(p);
     for ( int i = 0; i < 3; i++ ) {
         p->stages_built[i] = false;
         p->stages[i].clear( );
@@ -225,7 +255,8 @@ static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stag
 
 static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
                                      void* out, const void* image, float pre, float intra, float post, int first = 0,
-                                     int last = -1, const ArgmaxSlots* amax = nullptr, bool* amax_fused = nullptr);
+                                     int last = -1, const ArgmaxSlots* amax = nullptr, bool* amax_fused = nullptr,
+                                     float2* ws_a = nullptr, float2* ws_b = nullptr);
 
 static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, const void* image, float pre, float intra,
                                  float post, int first = 0, int last = -1, const ArgmaxSlots* amax = nullptr,
@@ -236,9 +267,10 @@ static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, c
     return run_stage_list(p, *sp, kOpNames[opi], in, out, image, pre, intra, post, first, last, amax, amax_fused);
 }
 
+// ws_a / ws_b (both or neither): workspace of an internal stage list instead of the plan's.
 static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
                                      void* out, const void* image, float pre, float intra, float post, int first, int last,
-                                     const ArgmaxSlots* amax, bool* amax_fused) {
+                                     const ArgmaxSlots* amax, bool* amax_fused, float2* ws_a, float2* ws_b) {
     if ( last < 0 )
         last = (int)stages.size( ) - 1;
     if ( first < 0 || last >= (int)stages.size( ) || first > last )
@@ -248,8 +280,8 @@ static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& s
     fastfft_status rc = ensure_device_state(p, &tw);
     if ( rc )
         return rc;
-    float2* wa = static_cast<float2*>(p->ws);
-    float2* wb = wa + p->ws_a;
+    float2* wa = ws_a ? ws_a : static_cast<float2*>(p->ws);
+    float2* wb = ws_a ? ws_b : wa + p->ws_a;
     auto    ptr = [&](Buf b) -> void* {
         switch ( b ) {
             case Buf::user_in: return in;
@@ -263,8 +295,8 @@ static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& s
     const bool half_in  = p->st.input_type == FASTFFT_INPUT_F16;
     const bool half_img = p->st.image_type == FASTFFT_IMAGE_C32H;
     auto       set_elem = [&](Elem el, bool& is_real, RealT& rt, CpxT& ct) {
-        is_real = el == Elem::real_input_type;
-        rt      = half_in ? RealT::f16 : RealT::f32;
+        is_real = el == Elem::real_input_type || el == Elem::real_f32;
+        rt      = (half_in && el != Elem::real_f32) ? RealT::f16 : RealT::f32;
         ct      = el == Elem::cpx_f32 ? CpxT::c64 : el == Elem::cpx_image_type ? (half_img ? CpxT::c32h : CpxT::c64)
                                                                             : (half_in ? CpxT::c32h : CpxT::c64);
     };
@@ -379,8 +411,8 @@ static fastfft_status fill_batch_items(fastfft_plan p, const std::vector<Stage>&
 }
 
 // Launches every chunk of the batch: R2C rows, fused columns, C2R rows (+ argmax keys).
-static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>& st, const void* image, float pre,
-                                       float intra, float post, const float2* tw) {
+static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>& st, const void* image, CpxT img_ct,
+                                       float pre, float intra, float post, const float2* tw) {
     const int   n    = (int)p->batch.size( );
     const int   C    = p->batch_chunk;
     const bool  half = p->st.input_type == FASTFFT_INPUT_F16;
@@ -392,7 +424,7 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
         if ( e == hipSuccess ) {
             StageIO io;
             io.img     = image;
-            io.img_ct  = p->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+            io.img_ct  = img_ct;
             io.scale   = intra;
             io.xitems  = p->batch_xc + c0;
             io.nxitems = cnt;
@@ -412,7 +444,7 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
 // Picks the chunk size: FASTFFT_BATCH_CHUNK if set, else the fastest of
 // 1, 2, 4, ... templates per launch on this batch (timed on a zero image), cached
 // per batch shape.  Allocates batch_dev for the chosen size and fills the descriptors.
-static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stage>& st) {
+static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stage>& st, CpxT img_ct) {
     const int n = (int)p->batch.size( );
     float2*   tw = nullptr;
     if ( auto rc = ensure_device_state(p, &tw) )
@@ -423,7 +455,7 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
     std::sort(sizes.begin( ), sizes.end( ));
     std::string key = std::to_string(st[1].N) + "/" + std::to_string(st[1].P) + "/" + std::to_string(st[0].N) + "/" +
                       std::to_string(st[2].M) + "x" + std::to_string(st[2].nlines) + "/" + std::to_string(st[1].o) + "/" +
-                      std::to_string(p->st.input_type) + std::to_string(p->st.image_type) + "/" + std::to_string(p->batch_peaks != nullptr);
+                      std::to_string(p->st.input_type) + std::to_string((int)img_ct) + "/" + std::to_string(p->batch_peaks != nullptr);
     for ( const auto& s : sizes )
         key += ":" + std::to_string(s.first) + "x" + std::to_string(s.second);
     int chosen = env_chunk( );
@@ -444,7 +476,7 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
         free_batch_device(p);
         hipError_t e = hipMalloc(&p->batch_dev, batch_dev_bytes(st, cmax, n));
         void*      img = nullptr;
-        const size_t img_bytes = (p->st.image_type == FASTFFT_IMAGE_C32H ? 4 : 8) * (size_t)st[1].gm.s1 * st[1].nlines;
+        const size_t img_bytes = (img_ct == CpxT::c32h ? 4 : 8) * (size_t)st[1].gm.s1 * st[1].nlines;
         if ( e == hipSuccess )
             e = hipMalloc(&img, img_bytes);
         if ( e == hipSuccess )
@@ -454,19 +486,20 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
             e = hipEventCreate(&e0);
         if ( e == hipSuccess )
             e = hipEventCreate(&e1);
-        float best = 0.f;
-        chosen     = 1;
+        float          best = 0.f;
+        fastfft_status run  = FASTFFT_SUCCESS; // a failed launch keeps its own message
+        chosen              = 1;
         for ( size_t ci = 0; e == hipSuccess && ci < cand.size( ); ci++ ) {
             p->batch_chunk = cand[ci];
-            if ( fill_batch_items(p, st, cand[ci]) != FASTFFT_SUCCESS ) {
-                e = hipErrorUnknown;
+            if ( (run = fill_batch_items(p, st, cand[ci])) != FASTFFT_SUCCESS )
                 break;
-            }
             float ms = 0.f;
-            for ( int rep = 0; rep < 2 && e == hipSuccess; rep++ ) {
+            for ( int rep = 0; rep < 2 && e == hipSuccess && ! run; rep++ ) {
                 e = hipEventRecord(e0, p->stream);
-                if ( e == hipSuccess && run_batch_chunks(p, st, img, 1.f, 1.f, 1.f, tw) != FASTFFT_SUCCESS )
-                    e = hipErrorUnknown;
+                if ( e == hipSuccess )
+                    run = run_batch_chunks(p, st, img, img_ct, 1.f, 1.f, 1.f, tw);
+                if ( run )
+                    break;
                 if ( e == hipSuccess )
                     e = hipEventRecord(e1, p->stream);
                 if ( e == hipSuccess )
@@ -474,6 +507,8 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
                 if ( e == hipSuccess )
                     e = hipEventElapsedTime(&ms, e0, e1);
             }
+            if ( run )
+                break;
             if ( e == hipSuccess && (ci == 0 || ms < best) ) {
                 best   = ms;
                 chosen = cand[ci];
@@ -486,6 +521,8 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
         if ( img )
             (void)hipFree(img);
         free_batch_device(p);
+        if ( run )
+            return run;
         if ( e != hipSuccess )
             return hip_fail(e, "batch chunk tuning");
         std::lock_guard<std::mutex> lock(g_chunk_mutex);
@@ -499,6 +536,132 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
     }
     p->batch_chunk = chosen;
     return fill_batch_items(p, st, chosen);
+}
+
+static int next_pow2(int v) {
+    int f = 1;
+    while ( f < v )
+        f <<= 1;
+    return f;
+}
+
+// Internal stage lists keep every operand in FP32: the planner's input-type
+// operands become FP32 ones.
+static void stages_to_f32(std::vector<Stage>& st) {
+    for ( Stage& s : st )
+        for ( Elem* e : {&s.src_elem, &s.dst_elem} ) {
+            if ( *e == Elem::real_input_type )
+                *e = Elem::real_f32;
+            else if ( *e == Elem::cpx_input_type )
+                *e = Elem::cpx_f32;
+        }
+}
+
+// Fills the derived fields of an internal plan's sizes (w = x/2+1 and the element
+// counts) as the plan calls do for a caller's plan.
+static void fill_sizes(PlanState& ps) {
+    ps.input_memory      = padded_memory_size(ps.fwd_in);
+    ps.fwd_output_memory = padded_memory_size(ps.fwd_out);
+    (void)padded_memory_size(ps.inv_in);
+    ps.inv_output_memory = padded_memory_size(ps.inv_out);
+    ps.compute_memory    = std::max(std::max(ps.input_memory, ps.fwd_output_memory),
+                                    std::max((long)padded_memory_size(ps.inv_in), ps.inv_output_memory));
+}
+
+static PlanState frame_plan(int input_type, const Dims& n, const Dims& out, const int origin[3]) {
+    PlanState ps;
+    ps.input_type = input_type;
+    ps.image_type = FASTFFT_IMAGE_C64;
+    ps.rank       = 2;
+    ps.fwd_set = ps.inv_set = true;
+    ps.fwd_in = ps.fwd_out = ps.inv_in = Dims{n.x, n.y, 1, 0};
+    ps.inv_out                          = Dims{out.x, out.y, 1, 0};
+    for ( int a = 0; a < 3; a++ )
+        ps.origin[a] = origin[a];
+    fill_sizes(ps);
+    return ps;
+}
+
+// Activates the sub-frame form of the batch (see fastfft_plan_s) when the frame
+// is at most a quarter of the plan's, FASTFFT_NO_SUBFRAME is unset and the
+// batched kernels cover the frame's correlation stages; otherwise leaves it off.
+static fastfft_status setup_subframe(fastfft_plan p) {
+    free_subframe(p);
+    const char* off = std::getenv("FASTFFT_NO_SUBFRAME");
+    if ( off && off[0] && std::atoi(off) != 0 )
+        return ok( );
+    const PlanState& st = p->st;
+    const Dims &     in = st.fwd_in, &nn = st.inv_in, &win = st.inv_out;
+    if ( st.transform_dimension != 2 )
+        return ok( );
+    const int fx = next_pow2(in.x + win.x - 1), fy = next_pow2(in.y + win.y - 1);
+    if ( fx >= nn.x || fy >= nn.y || 4L * fx * fy > (long)nn.x * nn.y )
+        return ok( );
+    const int zero[3] = {0, 0, 0};
+    // the templates' correlation on the frame: window origin 0 of the frame is lag origin of the plan
+    PlanState ps = st;
+    ps.fwd_out = ps.inv_in = Dims{fx, fy, 1, 0};
+    ps.image_type          = FASTFFT_IMAGE_C64;
+    for ( int a = 0; a < 3; a++ )
+        ps.origin[a] = 0;
+    fill_sizes(ps);
+    ps.validated = false;
+    std::vector<Stage> xc, inv, fwd;
+    if ( validate(ps).code || build_stages(ps, "xcorr", xc).code || xc.size( ) != 3 ||
+         ! batch_kernels_cover(xc[0], xc[1], xc[2], CpxT::c64) )
+        return ok( );
+    // the image reduced to the frame: trimmed inverse of the N-frame spectrum (read in the plan's image type) ...
+    PlanState pi = frame_plan(FASTFFT_INPUT_F32, nn, Dims{fx, fy, 1, 0}, st.origin);
+    // ... and the frame's spectrum
+    PlanState pf = frame_plan(FASTFFT_INPUT_F32, Dims{fx, fy, 1, 0}, Dims{fx, fy, 1, 0}, zero);
+    if ( validate(pi).code || build_stages(pi, "inv", inv).code || validate(pf).code || build_stages(pf, "fwd", fwd).code )
+        return ok( );
+    stages_to_f32(inv);
+    stages_to_f32(fwd);
+    inv.front( ).src_elem = Elem::cpx_image_type;
+    for ( const auto* list : {&inv, &fwd, &xc} )
+        for ( const Stage& s : *list )
+            if ( stage_lds_bytes(s.P) > kMaxLdsBytes )
+                return ok( );
+    // device block: frame, frame spectrum, and the larger workspace of the two internal lists
+    Dims   fd{fx, fy, 1, 0};
+    size_t s_floats = (size_t)padded_memory_size(fd);
+    size_t h_elems  = (size_t)(fx / 2 + 1) * (size_t)fy;
+    size_t ia, ib, fa, fb;
+    workspace_elems(pi, ia, ib);
+    workspace_elems(pf, fa, fb);
+    const size_t wa = std::max(ia, fa), wb = std::max(ib, fb);
+    s_floats        = (s_floats + 63) / 64 * 64;
+    const size_t bytes = sizeof(float) * s_floats + sizeof(float2) * (h_elems + wa + wb) + 256;
+    hipError_t   e     = hipMalloc(&p->sub_dev, bytes);
+    if ( e != hipSuccess ) {
+        p->sub_dev = nullptr;
+        return hip_fail(e, "hipMalloc(batch frame)");
+    }
+    p->sub_s      = static_cast<float*>(p->sub_dev);
+    p->sub_h      = reinterpret_cast<float2*>(p->sub_s + s_floats);
+    p->sub_wa     = p->sub_h + (h_elems + 1) / 2 * 2;
+    p->sub_wb     = p->sub_wa + (wa + 1) / 2 * 2;
+    p->sub_fx     = fx;
+    p->sub_fy     = fy;
+    p->sub_inv    = std::move(inv);
+    p->sub_fwd    = std::move(fwd);
+    p->sub_xc     = std::move(xc);
+    p->sub_active = true;
+    return ok( );
+}
+
+// The batch on the sub-frame: image -> frame -> frame spectrum, then every chunk.
+static fastfft_status run_subframe_batch(fastfft_plan p, const void* image, float pre, float intra, float post,
+                                         const float2* tw) {
+    const float frame_scale = 1.f / ((float)p->sub_fx * (float)p->sub_fy);
+    if ( auto rc = run_stage_list(p, p->sub_inv, "batch frame", const_cast<void*>(image), p->sub_s, nullptr, 1.f, 1.f,
+                                  frame_scale, 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+        return rc;
+    if ( auto rc = run_stage_list(p, p->sub_fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f, 0, -1,
+                                  nullptr, nullptr, p->sub_wa, p->sub_wb) )
+        return rc;
+    return run_batch_chunks(p, p->sub_xc, p->sub_h, CpxT::c64, pre, intra, post, tw);
 }
 
 extern "C" {
@@ -531,6 +694,7 @@ fastfft_status fastfft_destroy(fastfft_plan plan) {
     if ( plan->argmax_scratch )
         (void)hipFree(plan->argmax_scratch);
     free_batch_device(plan);
+    free_subframe(plan);
     delete plan;
     return ok( );
 }
@@ -738,6 +902,7 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
         lists[key] = std::move(st);
     }
     free_batch_device(plan);
+    free_subframe(plan);
     plan->batch        = std::move(items);
     plan->batch_stages = std::move(lists);
     plan->batch_peaks  = peaks_xy;
@@ -747,14 +912,19 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
     if ( auto rc = get_stages(plan, 2, &sp) )
         return rc;
     const CpxT img_ct = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
-    if ( n > 0 && sp->size( ) == 3 && batch_kernels_cover((*sp)[0], (*sp)[1], (*sp)[2], img_ct) ) {
-        if ( auto rc = setup_batch_kernels(plan, *sp) ) {
-            plan->batch.clear( );
-            plan->batch_stages.clear( );
-            return rc;
-        }
+    fastfft_status rc = ok( );
+    if ( n > 0 )
+        rc = setup_subframe(plan);
+    if ( ! rc && plan->sub_active )
+        rc = setup_batch_kernels(plan, plan->sub_xc, CpxT::c64);
+    else if ( ! rc && n > 0 && sp->size( ) == 3 && batch_kernels_cover((*sp)[0], (*sp)[1], (*sp)[2], img_ct) )
+        rc = setup_batch_kernels(plan, *sp, img_ct);
+    if ( rc ) {
+        plan->batch.clear( );
+        plan->batch_stages.clear( );
+        free_subframe(plan);
     }
-    return ok( );
+    return rc;
 }
 
 fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_to_search, fastfft_op pre_op,
@@ -778,7 +948,11 @@ fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_
         float2* tw = nullptr;
         if ( auto rc = ensure_device_state(plan, &tw) )
             return rc;
-        if ( auto rc = run_batch_chunks(plan, *sp, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), tw) )
+        const CpxT img_ct = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+        if ( auto rc = plan->sub_active ? run_subframe_batch(plan, image_to_search, op_scale(pre_op), op_scale(intra_op),
+                                                             op_scale(post_op), tw)
+                                        : run_batch_chunks(plan, *sp, image_to_search, img_ct, op_scale(pre_op),
+                                                           op_scale(intra_op), op_scale(post_op), tw) )
             return rc;
         if ( ! plan->batch.empty( ) ) {
             plan->last_buffer     = plan->batch.back( ).out;
@@ -965,13 +1139,29 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
         return fail(FASTFFT_INVALID_ARGUMENT, "describe: null argument");
     if ( auto rc = ensure_validated(plan) )
         return rc;
-    const int opi = op_index(op);
-    if ( opi < 0 )
-        return fail(FASTFFT_INVALID_ARGUMENT, std::string("unknown operation '") + op + "' (expected fwd, inv or xcorr)");
-    const std::vector<Stage>* sp = nullptr;
-    if ( auto rc = get_stages(plan, opi, &sp) )
-        return rc;
-    std::string s = describe(plan->st, *sp);
+    std::string s;
+    if ( std::strcmp(op, "batch") == 0 ) {
+        // the batch set by fastfft_set_batch: its form, chunk size and internal stage lists
+        s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk);
+        if ( plan->sub_active ) {
+            s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) + "\n";
+            s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +
+                 "templates:\n" + describe(plan->st, plan->sub_xc);
+        }
+        else {
+            s += " frame=full\n";
+        }
+    }
+    else {
+        const int opi = op_index(op);
+        if ( opi < 0 )
+            return fail(FASTFFT_INVALID_ARGUMENT,
+                        std::string("unknown operation '") + op + "' (expected fwd, inv, xcorr or batch)");
+        const std::vector<Stage>* sp = nullptr;
+        if ( auto rc = get_stages(plan, opi, &sp) )
+            return rc;
+        s = describe(plan->st, *sp);
+    }
     std::snprintf(buf, buf_len, "%s", s.c_str( ));
     if ( s.size( ) + 1 > buf_len )
         return fail(FASTFFT_INVALID_ARGUMENT, "describe: buffer too small");

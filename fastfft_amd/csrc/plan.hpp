@@ -34,8 +34,10 @@ enum class StageKind {
 // Buffers a stage reads or writes.
 enum class Buf { user_in, user_out, image, ws_a, ws_b };
 
-// Element storage of a stage operand.
-enum class Elem { real_input_type, cpx_input_type, cpx_f32, cpx_image_type };
+// Element storage of a stage operand.  The planner emits the first four;
+// real_f32 marks an FP32 real operand whatever the plan's input type (internal
+// stage lists whose intermediates stay FP32).
+enum class Elem { real_input_type, cpx_input_type, cpx_f32, cpx_image_type, real_f32 };
 
 struct Stage {
     StageKind   kind;

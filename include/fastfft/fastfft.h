@@ -181,7 +181,9 @@ fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, 
                                   void* out, fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op);
 
 /* Human-readable description of the stage sequence an operation would run
- * ("fwd", "inv" or "xcorr"), written without touching the GPU. */
+ * ("fwd", "inv" or "xcorr"), written without touching the GPU; "batch"
+ * describes the batch of fastfft_set_batch (its chunk size, and the frame and
+ * internal stage lists when it runs on a frame smaller than the plan's). */
 fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, size_t buf_len);
 
 /* Message of the last failed call on this thread ("" if none). */

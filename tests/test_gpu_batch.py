@@ -3,6 +3,12 @@
 Each output of a batch must agree with the single-template FwdImageInvFFT of the same template (the
 batched kernels use the sub-transform sizes of the largest template, so agreement is to rounding), each
 peak must equal the argmax of its output, and small batches are also checked against the FP64 oracle.  The batch is replayed from a HIP graph with eager work in between.
+
+Where the window and the largest template fit a frame of at most a quarter of the image (Fx >= Lx + Mx - 1,
+Fy >= Ly + My - 1), the batch reduces the image to that frame once per call and correlates every template
+on it; the "full_frame" mode switches that off, so both forms are compared with the same single
+correlations.  The image spectra are random complex numbers (not Hermitian on the DC and Nyquist
+columns), so the test also covers how the C2R pass treats those columns in both forms.
 """
 import numpy as np
 import pytest
@@ -40,11 +46,15 @@ def _setup(sizes, n, nout, origin, seed=0):
 
 
 BATCHES = [([(128, 128), (256, 256), (384, 384), (128, 128), (384, 384)], (4096, 4096), (512, 512), (-256, -256)),
+           ([(200, 200), (160, 120), (97, 200)], (2048, 2048), (150, 200), (-70, 30)),
            ([(40, 30), (64, 64), (17, 50)], (256, 128), (64, 32), (-32, -16)),
            ([(100, 60), (100, 60)], (256, 128), (100, 60), (-50, -30))]
 
+# frame of the batches that run on a reduced frame
+FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "512x512"}
 
-@pytest.mark.parametrize("mode", ["tuned", "chunk2", "per_template"])
+
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "full_frame", "per_template"])
 @pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
 def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
     """Batched kernels (chunk size from the tuner, or 2 to force several chunks) and the per-template
@@ -53,8 +63,13 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         monkeypatch.setenv("FASTFFT_BATCH_CHUNK", "2")
     if mode == "per_template":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    if mode == "full_frame":
+        monkeypatch.setenv("FASTFFT_NO_SUBFRAME", "1")
     H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
+    form = ft.Describe("batch").splitlines()[0]
+    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2") else None
+    assert f"frame={frame or 'full'}" in form, form
     ft.FwdImageInvFFTBatch(img)
     pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
     pitch = H.F.padded_pitch(nout[0])
@@ -73,9 +88,11 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
             assert O.rel_l2(got, want) < TOL, i
 
 
-def test_batch_graph_replays():
-    sizes, n, nout, origin = [(40, 30), (64, 64), (17, 50)], (256, 128), (64, 32), (-32, -16)
-    H.log("batch in a HIP graph, replayed with eager work in between")
+@pytest.mark.parametrize("case", [1, 2])
+def test_batch_graph_replays(case):
+    """Replays of a captured batch equal the eager batch, also for a batch on a reduced frame."""
+    sizes, n, nout, origin = BATCHES[case]
+    H.log(f"batch {sizes} in {n} -> {nout} in a HIP graph, replayed with eager work in between")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=9)
     side = torch.cuda.Stream()
     ft.SetStream(side.cuda_stream)
