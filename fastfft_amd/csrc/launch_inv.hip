@@ -2,6 +2,7 @@
 #include "full_wave.hpp"
 #include "launch_common.hpp"
 #include "rows_wave.hpp"
+#include "window_rows.hpp"
 
 namespace fastfft_amd {
 
@@ -175,6 +176,45 @@ static hipError_t try_full_inv(const Stage& s, const StageIO& io, const float2* 
     }
 }
 
+// Output-pruned inverse lines of N in {256, ..., 8192} with any window (P = 512,
+// or 256 for N = 256, whatever the stage's own sub-transform size).
+template <int P, int Q, typename TIn, bool HERM, typename TOut, bool REAL_OUT>
+static hipError_t go_window(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    k_inv_window_rows<P, Q, TIn, HERM, TOut, REAL_OUT><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+            static_cast<const TIn*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
+            s.nlines, io.scale, tw);
+    return hipGetLastError( );
+}
+
+static hipError_t try_window(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( env_int("FASTFFT_NO_WINDOW_ROWS", 0) || s.M < 1 || s.M > s.N || s.nlines < 1 )
+        return hipErrorNotSupported;
+    auto go = [&](auto pc, auto qc) -> hipError_t {
+        constexpr int P = decltype(pc)::value;
+        constexpr int Q = decltype(qc)::value;
+        if ( s.kind == StageKind::inv_c2r ) {
+            if ( io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real )
+                return hipErrorInvalidValue;
+            return io.dst_rt == RealT::f32 ? go_window<P, Q, float2, true, float, true>(s, io, tw, st)
+                                           : go_window<P, Q, float2, true, __half, true>(s, io, tw, st);
+        }
+        if ( io.src_real || io.dst_real || io.dst_ct != CpxT::c64 )
+            return hipErrorInvalidValue;
+        return io.src_ct == CpxT::c64 ? go_window<P, Q, float2, false, float2, false>(s, io, tw, st)
+                                      : go_window<P, Q, __half2, false, float2, false>(s, io, tw, st);
+    };
+    using I = std::integral_constant<int, 1>;
+    switch ( s.N ) {
+        case 256: return go(std::integral_constant<int, 256>{ }, I{ });
+        case 512: return go(std::integral_constant<int, 512>{ }, I{ });
+        case 1024: return go(std::integral_constant<int, 512>{ }, std::integral_constant<int, 2>{ });
+        case 2048: return go(std::integral_constant<int, 512>{ }, std::integral_constant<int, 4>{ });
+        case 4096: return go(std::integral_constant<int, 512>{ }, std::integral_constant<int, 8>{ });
+        case 8192: return go(std::integral_constant<int, 512>{ }, std::integral_constant<int, 16>{ });
+        default: return hipErrorNotSupported;
+    }
+}
+
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) && ! env_int("FASTFFT_NO_FULL_LINES", 0) && ! io.amax ) {
         hipError_t e = try_full_inv(s, io, tw, st);
@@ -184,6 +224,12 @@ hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipSt
     if ( s.kind == StageKind::inv_c2r && io.dst_real && ! io.src_real && io.src_ct == CpxT::c64 &&
          ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = io.dst_rt == RealT::f32 ? try_c2r_wave<float>(s, io, tw, st) : try_c2r_wave<__half>(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
+    // (an argmax asked of this C2R pass stays unfused: the caller runs it separately)
+    if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = try_window(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;
     }

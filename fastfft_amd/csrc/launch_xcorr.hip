@@ -5,6 +5,7 @@
 // N = P*Q <= 8192 and a contiguous, 16-byte aligned image column run the
 // workgroup-per-column kernel of xcorr_col.hpp; every other shape runs the
 // generic line kernel.
+#include <algorithm>
 #include <cstdint>
 
 #include "launch_common.hpp"
@@ -29,12 +30,23 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
 // One workgroup per column; columns 0 and nlines-1 share workgroup 0 when they
 // are the DC and Nyquist columns of a real transform (FASTFFT_XCORR_PACK=0
 // gives them a workgroup each, for tests).
+// Several templates per launch (io.xitems): when the columns alone give fewer
+// than about kXcorrGroupTarget workgroups, the templates are split into groups
+// along gridDim.y so that the launch still fills the GPU (each group stages its
+// column again; FASTFFT_XCORR_GROUPS overrides the count).
+constexpr int kXcorrGroupTarget = 4096;
+
 template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const int pack   = s.real_end_lines && s.nlines >= 2 && env_int("FASTFFT_XCORR_PACK", 1) ? 1 : 0;
     const int blocks = s.nlines - pack;
+    int       groups = 1;
+    if ( io.xitems && io.nxitems > 1 && blocks % 8 == 0 ) {
+        groups = env_int("FASTFFT_XCORR_GROUPS", (kXcorrGroupTarget + blocks - 1) / blocks);
+        groups = std::max(1, std::min(groups, io.nxitems));
+    }
     XcorrItem one{io.src, io.dst, s.L, 0};
-    k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks), dim3(64 * Q), 0, st>>>(
+    k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks, (unsigned)groups), dim3(64 * Q), 0, st>>>(
             one, static_cast<const XcorrItem*>(io.xitems), io.nxitems, s.im.s1, s.im.es, static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M,
             s.o, io.scale, tw, s.nlines - 1, pack);
     return hipGetLastError( );

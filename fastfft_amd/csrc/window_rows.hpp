@@ -1,0 +1,99 @@
+// Output-pruned inverse line transforms with one workgroup of Q wavefronts per
+// line, for windows the tuned C2R row kernel does not take (any origin, and up
+// to M = N outputs): the inverse y-pass of a trimmed InvFFT and its C2R x-pass
+// (the reference's block_fft_kernel_C2C_DECREASE inverse use and
+// C2R_DECREASE_XY, src/fastfft/FastFFT.cu:1405-1431, :1662-1692).
+//
+// With N = P*Q and U_q = IFFT_P( X[Q k' + q] ) (wave q, register Stockham),
+//   y[n] = sum_q W_N^{-n q} U_q[n mod P],   n = o + j, j < M,
+// evaluated per output by Horner's rule in w = W_N^{-n}:
+//   y[n] = U_0 + w (U_1 + w (U_2 + ... + w U_{Q-1})).
+// U_q is kept in LDS in natural order, so an output reads it at any rotation
+// (consecutive outputs read consecutive words: no bank conflicts), and one
+// twiddle per output comes from W_N^{o} W_N^{tid} W_N^{NT r}, output j = tid + NT r.
+// HERM: the line holds X[0 .. N/2] of a Hermitian sequence (X[N-k] = conj X[k]
+// above) and only the real part of y is stored (REAL_OUT).
+#pragma once
+
+#include "rows_wave.hpp"
+
+namespace fastfft_amd {
+
+template <int P, int Q, typename TIn, bool HERM, typename TOut, bool REAL_OUT>
+__global__ void __launch_bounds__(64 * Q)
+        k_inv_window_rows(const TIn* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o,
+                          int nlines, float post_scale, const float2* __restrict__ tw) {
+    constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
+    using FFT       = WaveFFT<P>;
+    static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
+    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_twn[NT];
+    __shared__ float2 s_w1[NT]; // W_N^k, k < NT
+    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t    = threadIdx.x % T;
+    const int     line = xcd_row(blockIdx.x, nlines);
+    const float2* twm  = multi_twiddles(tw);
+    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
+    s_w1[threadIdx.x] = twm[N + threadIdx.x];
+    // ---- the input line into LDS, decimated by Q, two values per lane
+    const TIn*    lp = in + im.base(line);
+    constexpr int KN = HERM ? H + 1 : N;
+    for ( int m = threadIdx.x; 2 * m < KN; m += NT ) {
+        lds[dec_slot<P, Q>(2 * m)] = load_cpx(lp, (long)(2 * m) * im.es);
+        if ( 2 * m + 1 < KN )
+            lds[dec_slot<P, Q>(2 * m + 1)] = load_cpx(lp, (long)(2 * m + 1) * im.es);
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int k = Q * (t + T * i) + q;
+        if constexpr ( HERM ) {
+            const bool   lo = k <= H;
+            const float2 z  = lds[dec_slot<P, Q>(lo ? k : N - k)];
+            v[i]            = lo ? z : make_float2(z.x, -z.y);
+        }
+        else {
+            v[i] = lds[dec_slot<P, Q>(k)];
+        }
+    }
+    __syncthreads( );
+    float2* wl = lds + q * FFT::SLOTS;
+    fft.template run<+1>(v, wl, t);
+    // U_q[m], m = t + 64 i, in natural order in this wave's own region
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        wl[t + T * i] = v[i];
+    __syncthreads( );
+    // ---- the window: Horner over q for each output
+    const int    oN    = ((o % N) + N) % N;
+    const int    oP    = oN & (P - 1);
+    const float2 w_o   = twm[N + oN];          // W_N^{o}
+    const float2 w_tid = s_w1[threadIdx.x];   // W_N^{tid}
+    TOut*        op    = out + om.base(line);
+    for ( int r = 0; r * NT < M; r++ ) {
+        const int j = (int)threadIdx.x + NT * r;
+        if ( j >= M )
+            break;
+        const float2 wr = twm[N + ((NT * r) & (N - 1))]; // wave-uniform
+        const float2 w  = cmul(cmul(w_o, w_tid), wr);     // W_N^{o + j}
+        const int    m  = (oP + j) & (P - 1);
+        float2       acc = lds[(Q - 1) * FFT::SLOTS + m];
+#pragma unroll
+        for ( int qq = Q - 2; qq >= 0; qq-- ) {
+            const float2 u = lds[qq * FFT::SLOTS + m];
+            acc            = cmulc(acc, w); // acc * conj(w) = acc * W_N^{-(o + j)}
+            acc.x += u.x;
+            acc.y += u.y;
+        }
+        if constexpr ( REAL_OUT )
+            store_real(op, (long)j * om.es, acc.x * post_scale);
+        else
+            store_cpx(op, (long)j * om.es, make_float2(acc.x * post_scale, acc.y * post_scale));
+    }
+}
+
+} // namespace fastfft_amd

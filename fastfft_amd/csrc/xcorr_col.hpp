@@ -60,10 +60,12 @@ struct ImgStage<__half2> {
 };
 
 // Grid: nlines - 1 workgroups when pack_ends (columns 0 and last_line share
-// workgroup 0), else nlines.  img[c*ldg + n] is element n of image column c
-// (img and img + ldg 16-byte aligned).  The workgroup correlates its column
-// with every template of the launch: `one`, or items[0 .. nitems) when items is
-// not null.  The image column is read from memory once per launch and kept in
+// workgroup 0), else nlines, times gridDim.y template groups.  img[c*ldg + n]
+// is element n of image column c (img and img + ldg 16-byte aligned).  The
+// workgroup correlates its column with the templates of the launch: `one`, or
+// its group's share of items[0 .. nitems) when items is not null (group b of
+// G takes ceil(nitems / G) consecutive items; gridDim.x a multiple of 8 keeps
+// the column-to-XCD mapping of the one-group grid).  The image column is read from memory once per launch and kept in
 // registers (in LDS for the packed end columns) for all templates.  With
 // a_es, o_es > 1 (row-major template spectra and outputs, as the x-pass kernels
 // prefer) the template gather and the output scatter are the only strided
@@ -75,7 +77,7 @@ struct ImgStage<__half2> {
 // W_N^{64 i q} are scalar loads, so the only vector memory traffic is the
 // column data.
 template <int P, int Q, int RS, typename TImg>
-__global__ void __launch_bounds__(64 * Q)
+__global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <= 512 ? 4 : 1)))
         k_xcorr_col(XcorrItem one, const XcorrItem* __restrict__ items, int nitems, long lda, long a_es,
                     const TImg* __restrict__ img, long ldg, long ldo, long o_es, int M, int o, float intra_scale,
                     const float2* __restrict__ tw, int last_line, int pack_ends) {
@@ -96,8 +98,17 @@ __global__ void __launch_bounds__(64 * Q)
     const long     col  = xcd_row(blockIdx.x, gridDim.x);
     const bool     ends = pack_ends && col == 0;
     const float2*  twm  = multi_twiddles(tw);
-    if ( ! items )
+    if ( ! items ) {
         nitems = 1;
+    }
+    else if ( gridDim.y > 1 ) {
+        const int per   = (nitems + (int)gridDim.y - 1) / (int)gridDim.y;
+        const int first = (int)blockIdx.y * per;
+        items += first;
+        nitems = min(per, nitems - first);
+        if ( nitems <= 0 ) // the whole workgroup leaves before any barrier
+            return;
+    }
 
     // ---- stage: twiddle tables, image column
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);

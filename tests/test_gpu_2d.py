@@ -229,14 +229,21 @@ def test_fwd_random(fin, fout, mode, monkeypatch):
 
 INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256, -256)), ((4096, 4096), (512, 512), (0, 0)),
              ((256, 128), (100, 60), (-50, -30)), ((64, 2048), (64, 300), (0, 7)), ((8192, 32), (8192, 32), (0, 0)),
-             ((1024, 1024), (1000, 1000), (3, -5))]
+             ((1024, 1024), (1000, 1000), (3, -5)),
+             # windowed inverse rows: M > P = 512, origins off the 64-lane grid, non-square windows
+             ((4096, 4096), (1024, 1024), (-256, -300)), ((2048, 2048), (700, 96), (13, -77)),
+             ((512, 256), (300, 256), (-7, 0))]
 
 
-@pytest.mark.parametrize("mode", ["default", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_window", "generic"])
 @pytest.mark.parametrize("nin,nout,origin", INV_CASES)
 def test_inv_random(nin, nout, origin, mode, monkeypatch):
+    """Trimmed inverses against the FP64 oracle with the default kernel choice, without the windowed
+    inverse-row kernel (the generic line kernel takes its shapes), and with the generic kernel only."""
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    if mode == "no_window":
+        monkeypatch.setenv("FASTFFT_NO_WINDOW_ROWS", "1")
     H.log(f"inv random {nin} -> {nout} at {origin} mode={mode}")
     rng = np.random.default_rng(nin[0] + nout[1])
     w = nin[0] // 2 + 1
