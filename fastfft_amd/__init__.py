@@ -30,6 +30,7 @@ _STATUS_NAMES = ["SUCCESS", "INVALID_ARGUMENT", "INVALID_DIMENSION", "UNSUPPORTE
 INPUT_F32, INPUT_F16 = 0, 1
 IMAGE_C64, IMAGE_C32H = 0, 1
 OP_NOOP, OP_SCALE, OP_CONJ_MUL, OP_CONJ_MUL_THEN_SCALE = 0, 1, 2, 3
+PRECISION_FP32, PRECISION_SPLIT_BF16 = 0, 1
 
 
 class FastFFTError(RuntimeError):
@@ -83,6 +84,7 @@ def lib():
         "fastfft_set_inv_plan": [P, S, S, S, S, S, S, I],
         "fastfft_set_output_window": [P, I, I, I],
         "fastfft_set_stream": [P, V],
+        "fastfft_set_precision": [P, ctypes.c_int],
         "fastfft_fwd": [P, V, V, _Op, _Op],
         "fastfft_inv": [P, V, V, _Op, _Op],
         "fastfft_fwd_image_inv": [P, V, V, V, _Op, _Op, _Op],
@@ -202,6 +204,12 @@ class FourierTransformer:
             raw = stream.cuda_stream
         _check(lib().fastfft_set_stream(self._h, raw))
         self._stream_set = True
+
+    def SetPrecision(self, precision) -> None:
+        """Arithmetic of the correlation pass: "fp32" (default) or "split_bf16" (matrix-core DFT-32
+        stages with hi + lo BF16 operands where they apply; see fastfft_set_precision)."""
+        code = {"fp32": PRECISION_FP32, "split_bf16": PRECISION_SPLIT_BF16}.get(precision, precision)
+        _check(lib().fastfft_set_precision(self._h, int(code)))
 
     def _bind_torch_stream(self) -> None:
         if self._stream_set:

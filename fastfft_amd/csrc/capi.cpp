@@ -70,7 +70,8 @@ hipError_t twiddles_for_current_device(float2** out, hipStream_t stream) {
 
 struct fastfft_plan_s {
     PlanState   st;
-    hipStream_t stream = hipStreamPerThread;
+    hipStream_t stream    = hipStreamPerThread;
+    int         precision = FASTFFT_PRECISION_FP32;
     void*       ws     = nullptr; // workspace: ws_a then ws_b (float2)
     size_t      ws_a = 0, ws_b = 0;
     int         ws_device = -1;
@@ -326,7 +327,8 @@ static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& s
                 e = launch_inv(s, io, tw, p->stream);
                 break;
             case StageKind::xcorr:
-                io.scale = intra;
+                io.scale      = intra;
+                io.split_bf16 = p->precision == FASTFFT_PRECISION_SPLIT_BF16;
                 e        = launch_xcorr(s, io, tw, p->stream);
                 break;
             default: e = hipErrorInvalidValue;
@@ -426,8 +428,9 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
             io.img     = image;
             io.img_ct  = img_ct;
             io.scale   = intra;
-            io.xitems  = p->batch_xc + c0;
-            io.nxitems = cnt;
+            io.xitems     = p->batch_xc + c0;
+            io.nxitems    = cnt;
+            io.split_bf16 = p->precision == FASTFFT_PRECISION_SPLIT_BF16;
             e          = launch_xcorr(st[1], io, tw, p->stream);
         }
         if ( e == hipSuccess )
@@ -455,7 +458,8 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
     std::sort(sizes.begin( ), sizes.end( ));
     std::string key = std::to_string(st[1].N) + "/" + std::to_string(st[1].P) + "/" + std::to_string(st[0].N) + "/" +
                       std::to_string(st[2].M) + "x" + std::to_string(st[2].nlines) + "/" + std::to_string(st[1].o) + "/" +
-                      std::to_string(p->st.input_type) + std::to_string((int)img_ct) + "/" + std::to_string(p->batch_peaks != nullptr);
+                      std::to_string(p->st.input_type) + std::to_string((int)img_ct) + "/" + std::to_string(p->batch_peaks != nullptr) +
+                      "/" + std::to_string(p->precision);
     for ( const auto& s : sizes )
         key += ":" + std::to_string(s.first) + "x" + std::to_string(s.second);
     int chosen = env_chunk( );
@@ -761,6 +765,15 @@ fastfft_status fastfft_set_stream(fastfft_plan plan, void* stream) {
     if ( auto rc = check_plan(plan) )
         return rc;
     plan->stream = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+    return ok( );
+}
+
+fastfft_status fastfft_set_precision(fastfft_plan plan, int precision) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( precision != FASTFFT_PRECISION_FP32 && precision != FASTFFT_PRECISION_SPLIT_BF16 )
+        return fail(FASTFFT_INVALID_ARGUMENT, "precision must be FASTFFT_PRECISION_FP32 or FASTFFT_PRECISION_SPLIT_BF16");
+    plan->precision = precision;
     return ok( );
 }
 
@@ -1142,7 +1155,8 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
     std::string s;
     if ( std::strcmp(op, "batch") == 0 ) {
         // the batch set by fastfft_set_batch: its form, chunk size and internal stage lists
-        s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk);
+        s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk) +
+            " precision=" + (plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? "split_bf16" : "fp32");
         if ( plan->sub_active ) {
             s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) + "\n";
             s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +

@@ -117,6 +117,59 @@ __device__ __forceinline__ void store_cpx<float2>(float2* p, long i, float2 v) {
 template <>
 __device__ __forceinline__ void store_cpx<__half2>(__half2* p, long i, float2 v) { p[i] = __float22half2_rn(v); }
 
+// ---------------------------------------------------------------- global-memory pointers
+// A pointer that a kernel reads from memory (a launch descriptor) is generic to
+// the compiler, which then accesses it with flat instructions.  Those count on
+// the LDS counter too, so every LDS wait of the kernel would also wait for the
+// outstanding flat loads and stores.  Accesses through global_t pointers are
+// global loads and stores.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ global_t<T>* as_global(T* p) {
+    return (global_t<T>*)p;
+}
+
+// (half precision values move as their bit patterns: the half types do not
+// copy from an address-space-qualified object)
+__device__ __forceinline__ float load_real(const global_t<float>* p, long i) { return p[i]; }
+__device__ __forceinline__ float load_real(const global_t<__half>* p, long i) {
+    return __half2float(__ushort_as_half(reinterpret_cast<const global_t<unsigned short>*>(p)[i]));
+}
+__device__ __forceinline__ void store_real(global_t<float>* p, long i, float v) { p[i] = v; }
+__device__ __forceinline__ void store_real(global_t<__half>* p, long i, float v) {
+    reinterpret_cast<global_t<unsigned short>*>(p)[i] = __half_as_ushort(__float2half_rn(v));
+}
+// float2 / float4 move as native vectors (the HIP vector classes do not copy
+// from an address-space-qualified object either)
+typedef float gvec2 __attribute__((ext_vector_type(2)));
+typedef float gvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 load_cpx(const global_t<float2>* p, long i) {
+    const gvec2 v = reinterpret_cast<const global_t<gvec2>*>(p)[i];
+    return make_float2(v.x, v.y);
+}
+// elements 2 m and 2 m + 1 of a 16-byte aligned line, as one 16-byte access
+__device__ __forceinline__ float4 load_cpx_pair(const global_t<float2>* p, long m) {
+    const gvec4 v = reinterpret_cast<const global_t<gvec4>*>(p)[m];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_cpx_pair(global_t<float2>* p, long m, float2 a, float2 b) {
+    reinterpret_cast<global_t<gvec4>*>(p)[m] = gvec4{a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ float2 load_cpx(const global_t<__half2>* p, long i) {
+    const unsigned u = reinterpret_cast<const global_t<unsigned>*>(p)[i];
+    return make_float2(__half2float(__ushort_as_half((unsigned short)(u & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(u >> 16))));
+}
+__device__ __forceinline__ void store_cpx(global_t<float2>* p, long i, float2 v) {
+    reinterpret_cast<global_t<gvec2>*>(p)[i] = gvec2{v.x, v.y};
+}
+__device__ __forceinline__ void store_cpx(global_t<__half2>* p, long i, float2 v) {
+    const unsigned lo = __half_as_ushort(__float2half_rn(v.x)), hi = __half_as_ushort(__float2half_rn(v.y));
+    reinterpret_cast<global_t<unsigned>*>(p)[i] = lo | (hi << 16);
+}
+
 // ---------------------------------------------------------------- small DFTs in registers
 // In-place radix-R DFT (R = 2,4,8,16) by compile-time radix-2 decimation in time.
 // Constant twiddles W_R^k are exact decimal expansions; multiplications by 1 and

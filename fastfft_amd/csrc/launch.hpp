@@ -37,6 +37,9 @@ struct StageIO {
     // array of XcorrItem, xcorr_col.hpp) instead of src / dst
     const void* xitems  = nullptr;
     int         nxitems = 0;
+    // xcorr only: run the split-BF16 matrix-core kernel where it covers the stage
+    // (xcorr_mfma_covers), the FP32 kernels elsewhere
+    bool split_bf16 = false;
 };
 
 // Batched correlation over n templates with the stage parameters of `s` (the
@@ -46,6 +49,9 @@ struct StageIO {
 bool       r2c_rows_batch_covers(const Stage& s);
 bool       c2r_rows_batch_covers(const Stage& s);
 bool       xcorr_col_covers(const Stage& s, CpxT img_ct);
+// the split-BF16 matrix-core correlation pass (xcorr_mfma.hpp): N = 1024, L <= 512,
+// window origin 0, single-index lines with unit line stride, FP32 image
+bool       xcorr_mfma_covers(const Stage& s, CpxT img_ct);
 bool       batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r, CpxT img_ct);
 hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
                                  const float2* tw, hipStream_t stream);

@@ -10,6 +10,7 @@
 
 #include "launch_common.hpp"
 #include "xcorr_col.hpp"
+#include "xcorr_mfma.hpp"
 
 namespace fastfft_amd {
 
@@ -95,6 +96,28 @@ bool xcorr_col_covers(const Stage& s, CpxT img_ct) {
     return rs == 0 || rs == E / 2;
 }
 
+bool xcorr_mfma_covers(const Stage& s, CpxT img_ct) {
+    return s.kind == StageKind::xcorr && s.N == 1024 && s.L >= 1 && s.L <= 512 && s.o == 0 && s.M >= 1 && s.M <= 1024 &&
+           img_ct == CpxT::c64 && single_index_lines(s.im) && s.im.s1 == 1 && single_index_lines(s.om) && s.om.s1 == 1 &&
+           single_index_lines(s.gm) && s.gm.es == 1;
+}
+
+// Template groups per column: about 2048 workgroups in all, at least 4 templates
+// per group where the batch has them (the workgroup set-up is shared by its
+// group); FASTFFT_MFMA_GROUPS overrides.
+static hipError_t go_xcorr_mfma(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const int n      = io.xitems ? io.nxitems : 1;
+    int       groups = env_int("FASTFFT_MFMA_GROUPS", 0);
+    if ( groups <= 0 )
+        groups = std::min((2048 + s.nlines - 1) / s.nlines, (n + kMfmaWaves - 1) / kMfmaWaves);
+    groups = std::max(1, std::min(groups, n));
+    XcorrItem one{io.src, io.dst, s.L, 0};
+    k_xcorr_mfma1024<<<dim3((unsigned)s.nlines, (unsigned)groups), dim3(256), 0, st>>>(
+            one, static_cast<const XcorrItem*>(io.xitems), n, s.im.es, static_cast<const float2*>(io.img), s.gm.s1,
+            s.om.es, s.M, io.scale, tw);
+    return hipGetLastError( );
+}
+
 bool batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r, CpxT img_ct) {
     return r2c.kind == StageKind::fwd_real && xcorr.kind == StageKind::xcorr && c2r.kind == StageKind::inv_c2r &&
            r2c_rows_batch_covers(r2c) && xcorr_col_covers(xcorr, img_ct) && c2r_rows_batch_covers(c2r) &&
@@ -102,6 +125,8 @@ bool batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r,
 }
 
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( io.split_bf16 && xcorr_mfma_covers(s, io.img_ct) )
+        return go_xcorr_mfma(s, io, tw, st);
     // 16-byte image loads: aligned image base (the column starts are checked with the shape)
     const bool col_ok = xcorr_col_covers(s, io.img_ct) && reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 &&
                         ! env_int("FASTFFT_NO_WAVE_XCORR", 0);

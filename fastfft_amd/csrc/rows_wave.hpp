@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(64 * Q)
         L     = it.L;
     }
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    const TIn* rp = in + im.base(row);
+    const auto* rp = as_global(in) + im.base(row);
     for ( int n = threadIdx.x; n < L; n += NT )
         s_row[n] = load_real(rp, n * im.es) * pre_scale;
     __syncthreads( );
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(64 * Q)
         v[i]           = cmul(make_float2(x * c.x, x * c.y), lw);
     }
     fft.template run<-1>(v, lds + q * FFT::SLOTS, t);
-    float2* op = out + om.base(row);
+    auto* op = as_global(out) + om.base(row);
     if ( om.es == 1 ) {
         // row order through LDS, then 16-byte stores (op is 16-byte aligned)
         __syncthreads( );
@@ -122,10 +122,10 @@ __global__ void __launch_bounds__(64 * Q)
             const float2 u0 = lds[dec_slot<P, Q>(2 * m)];
             if ( 2 * m + 1 < nout ) {
                 const float2 u1                      = lds[dec_slot<P, Q>(2 * m + 1)];
-                reinterpret_cast<float4*>(op)[m] = make_float4(u0.x, u0.y, u1.x, u1.y);
+                store_cpx_pair(op, m, u0, u1);
             }
             else {
-                op[2 * m] = u0;
+                store_cpx(op, 2 * m, u0);
             }
         }
     }
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(64 * Q)
         for ( int i = 0; i < E; i++ ) {
             const int k = Q * (t + T * i) + q;
             if ( k < nout )
-                op[k * om.es] = v[i];
+                store_cpx(op, (long)k * om.es, v[i]);
         }
     }
 }
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(64 * Q)
         row_keys              = it.keys;
     }
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    const float2* cp     = in + im.base(row);
+    const auto*   cp     = as_global(in) + im.base(row);
     const bool    staged = im.es == 1;
     if ( staged ) {
         // two values per lane (the decimated layout is conflict-free for pairs),
@@ -191,13 +191,13 @@ __global__ void __launch_bounds__(64 * Q)
         for ( int m = threadIdx.x; 2 * m <= H; m += NT ) {
             float2 u0, u1;
             if ( a16 && 2 * m + 1 <= H ) {
-                const float4 f = reinterpret_cast<const float4*>(cp)[m];
+                const float4 f = load_cpx_pair(cp, m);
                 u0             = make_float2(f.x, f.y);
                 u1             = make_float2(f.z, f.w);
             }
             else {
-                u0 = cp[2 * m];
-                u1 = 2 * m + 1 <= H ? cp[2 * m + 1] : make_float2(0.f, 0.f);
+                u0 = load_cpx(cp, 2 * m);
+                u1 = 2 * m + 1 <= H ? load_cpx(cp, 2 * m + 1) : make_float2(0.f, 0.f);
             }
             lds[dec_slot<P, Q>(2 * m)] = u0;
             if ( 2 * m + 1 <= H )
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(64 * Q)
         const int  k  = Q * (t + T * i) + q;
         const bool lo = k <= H;
         const int  kk = lo ? k : N - k;
-        float2     z  = staged ? lds[dec_slot<P, Q>(kk)] : cp[kk * im.es];
+        float2     z  = staged ? lds[dec_slot<P, Q>(kk)] : load_cpx(cp, (long)kk * im.es);
         if ( ! lo )
             z.y = -z.y;
         v[i] = z;
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(64 * Q)
         red[t + T * i] = u.x * w.x + u.y * w.y;
     }
     __syncthreads( );
-    TOut*              op   = out + om.base(row);
+    auto*              op   = as_global(out) + om.base(row);
     unsigned long long best = 0;
     for ( int j = threadIdx.x; j < M; j += NT ) {
         float s2 = 0.f;
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(64 * Q)
     if ( row_keys ) {
         best = block_max_key(best);
         if ( threadIdx.x == 0 )
-            row_keys[blockIdx.x] = best;
+            as_global(row_keys)[blockIdx.x] = best;
     }
 }
 

@@ -160,15 +160,15 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
         }
         for ( int it = 0; it < nitems; it++ ) {
             const XcorrItem item = items ? items[it] : one;
-            const float2*   ta   = static_cast<const float2*>(item.a);
-            float2*         to   = static_cast<float2*>(item.out);
+            const auto*     ta   = as_global(static_cast<const float2*>(item.a));
+            auto*           to   = as_global(static_cast<float2*>(item.out));
             // the previous template's reduction (or the image reads) are done with LDS
             __syncthreads( );
             {
-                const float2* ac = ta + col * lda;
-                const float2* a1 = ta + (long)last_line * lda;
+                const auto* ac = ta + col * lda;
+                const auto* a1 = ta + (long)last_line * lda;
                 for ( int n = threadIdx.x; n < item.L; n += NT )
-                    s_tpl[n] = ENDS ? make_float2(ac[n * a_es].x, a1[n * a_es].x) : ac[n * a_es];
+                    s_tpl[n] = ENDS ? make_float2(load_cpx(ac, n * a_es).x, load_cpx(a1, n * a_es).x) : load_cpx(ac, n * a_es);
             }
             __syncthreads( );
             // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
@@ -199,8 +199,8 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
                     wl[FFT::Layout::slot(t + T * i)] = v[i];
                 __syncthreads( );
                 const int   qp = (Q - q) % Q;
-                const TImg* g0 = img;
-                const TImg* g1 = img + (long)last_line * ldg;
+                const auto* g0 = as_global(img);
+                const auto* g1 = as_global(img) + (long)last_line * ldg;
                 int         tt = t;
                 asm volatile("" : "+s"(g0), "+s"(g1), "+v"(tt));
 #pragma unroll
@@ -242,11 +242,11 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
                 for ( int w = 0; w < Q; w++ )
                     acc = cadd(acc, lds[w * FFT::SLOTS + sj]);
                 if constexpr ( ! ENDS ) {
-                    to[col * ldo + j * o_es] = acc;
+                    store_cpx(to, col * ldo + j * o_es, acc);
                 }
                 else {
-                    to[j * o_es]                         = make_float2(acc.x, 0.f);
-                    to[(long)last_line * ldo + j * o_es] = make_float2(acc.y, 0.f);
+                    store_cpx(to, j * o_es, make_float2(acc.x, 0.f));
+                    store_cpx(to, (long)last_line * ldo + j * o_es, make_float2(acc.y, 0.f));
                 }
             }
         }

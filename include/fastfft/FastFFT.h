@@ -177,6 +177,12 @@ class FourierTransformer {
     // Extension: stream for all work (default: the per-thread default stream).
     void SetStream(hipStream_t stream) { detail::check(fastfft_set_stream(plan_, stream), "SetStream"); }
 
+    // Extension: arithmetic of the correlation pass (FASTFFT_PRECISION_FP32 or
+    // FASTFFT_PRECISION_SPLIT_BF16, see fastfft_set_precision).
+    void SetPrecision(fastfft_precision precision) {
+        detail::check(fastfft_set_precision(plan_, (int)precision), "SetPrecision");
+    }
+
     inline void Wait( ) { detail::check(fastfft_wait(plan_), "Wait"); }
 
     auto inline GetDeviceBufferPointer( ) {

@@ -47,6 +47,12 @@ typedef enum fastfft_image_type { FASTFFT_IMAGE_C64 = 0 /* float2 */, FASTFFT_IM
 
 /* Element-wise operators fused into the transforms; the reference's functors
  * KernelFunction::my_functor<T, N, IKF_t> (include/detail/functors.h:15-80). */
+/* Arithmetic of the correlation pass (fastfft_set_precision). */
+typedef enum fastfft_precision {
+    FASTFFT_PRECISION_FP32       = 0, /* FP32 radix passes (default, the drop-in path) */
+    FASTFFT_PRECISION_SPLIT_BF16 = 1  /* DFT-32 stages as split (hi + lo) BF16 matrix products, FP32 accumulation */
+} fastfft_precision;
+
 typedef enum fastfft_op_kind {
     FASTFFT_OP_NOOP                = 0,
     FASTFFT_OP_SCALE               = 1, /* pre/post op: x * scale */
@@ -93,6 +99,14 @@ fastfft_status fastfft_set_output_window(fastfft_plan plan, int origin_x, int or
  * cudaStreamPerThread, docs/_docs/references/usage.md:13).  stream is a
  * hipStream_t; NULL selects the per-thread default stream. */
 fastfft_status fastfft_set_stream(fastfft_plan plan, void* stream);
+
+/* Extension, no counterpart in the reference: arithmetic of the fused
+ * correlation pass.  FASTFFT_PRECISION_SPLIT_BF16 runs the pass on the matrix
+ * cores where it applies (1024-point second axis, at most 512 template rows,
+ * window origin 0: the frames of a batched correlation whose window and
+ * templates fit 1024 x 1024) with about 1e-5 relative error; every other pass
+ * stays FP32.  Takes effect for the next transform or fastfft_set_batch. */
+fastfft_status fastfft_set_precision(fastfft_plan plan, int precision);
 
 /* FwdFFT(input, output = nullptr, pre_op, intra_op) (FastFFT.cu:263-277, :315-459).
  * out == NULL transforms in place. */

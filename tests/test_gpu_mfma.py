@@ -1,0 +1,158 @@
+"""Split-BF16 matrix-core form of the batched correlation (fastfft_set_precision).
+
+The correlation pass of each 1024 x 1024 frame runs its DFT-32 stages as hi + lo BF16 matrix products
+with FP32 accumulation; every other pass stays FP32.  Outputs are compared with the FP64 oracle of the
+full 4096 x 4096 pipeline (relative L2 within 2e-5), and every peak must equal the FP32 path's, the
+oracle's and the planted lag.  The FP16 I/O case has the shape of 512 x 512 FP16 templates (radius-220
+masks) against a 4096 x 4096 FP16 micrograph; there FP16 storage sets the error bound (2e-3, as for the
+FP32 path).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from . import gpu_helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL_BF16 = 2e-5
+TOL_F16 = 2e-3
+
+
+def _planted(n, t, m, radius, count, seed):
+    """Micrograph noise plus 0.5 x template i at a seeded lag inside the centred m x m window."""
+    from fastfft_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    tmpls = [W.masked_template(t, radius, seed=seed + 1 + i) for i in range(count)]
+    mic = rng.standard_normal((n, n)).astype(np.float32)
+    lags = []
+    for tp in tmpls:
+        lx, ly = int(rng.integers(-m // 2 + 8, m // 2 - 8)), int(rng.integers(-m // 2 + 8, m // 2 - 8))
+        lags.append((lx, ly))
+        ys, xs = (np.arange(t) + ly) % n, (np.arange(t) + lx) % n
+        mic[np.ix_(ys, xs)] += 0.5 * tp
+    return mic, tmpls, lags
+
+
+def _batch(ft, tbufs, t, n_out_elems, precision, torch, d):
+    outs = [H.dev_zeros(n_out_elems) for _ in tbufs]
+    peaks = torch.full((2 * len(tbufs),), -1, dtype=torch.int32, device=d)
+    vals = torch.zeros(len(tbufs), dtype=torch.float32, device=d)
+    ft.SetPrecision(precision)
+    ft.SetBatch(tbufs, [(t, t)] * len(tbufs), outs, peaks, vals)
+    return outs, peaks, vals
+
+
+def test_split_bf16_batch_4096_384_against_oracle():
+    from fastfft_amd import workloads as W
+
+    n, t, m = 4096, 384, 512
+    H.log("split-BF16 batch 4096^2 x 384^2 -> 512^2 against the FP64 oracle")
+    torch, d = H.torch_dev()
+    mic, tmpls, lags = _planted(n, t, m, 160.0, 4, seed=21)
+    mft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1))
+    mbuf = H.dev_zeros(mft.ReturnFwdOutputMemorySize())
+    mbuf[:] = H.to_dev(W.to_padded(mic).reshape(-1))
+    mft.FwdFFT(mbuf)
+    mspec = H.complex_from_f32(H.to_host(mbuf), (n // 2 + 1, n))
+    tbufs = [H.to_dev(W.to_padded(tp).reshape(-1)) for tp in tmpls]
+    ft = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
+    outs, peaks, vals = _batch(ft, tbufs, t, ft.ReturnInvOutputMemorySize(), "split_bf16", torch, d)
+    form = ft.Describe("batch").splitlines()[0]
+    assert "precision=split_bf16" in form and "frame=1024x1024" in form, form
+    ft.FwdImageInvFFTBatch(mbuf)
+    ref = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
+    outs32, peaks32, _ = _batch(ref, tbufs, t, ref.ReturnInvOutputMemorySize(), "fp32", torch, d)
+    ref.FwdImageInvFFTBatch(mbuf)
+    pk, pk32 = H.to_host(peaks).reshape(-1, 2), H.to_host(peaks32).reshape(-1, 2)
+    for i, tp in enumerate(tmpls):
+        got = H.real_window(H.to_host(outs[i]), m, m)
+        want = O.xcorr_2d(W.to_padded(tp), t, mspec, n, n, m, m, -m // 2, -m // 2)
+        err = O.rel_l2(got, want)
+        err32 = O.rel_l2(H.real_window(H.to_host(outs32[i]), m, m), want)
+        print(f"template {i}: split-BF16 rel_l2={err:.3e} (FP32 path {err32:.3e})")
+        assert err < TOL_BF16, i
+        wx, wy, _ = O.argmax_2d(want)
+        assert tuple(pk[i]) == (wx, wy) == tuple(pk32[i]), i
+        assert (wx - m // 2, wy - m // 2) == lags[i], i
+    # replays of a captured batch equal the eager batch
+    want_outs = [H.to_host(o).copy() for o in outs]
+    side = torch.cuda.Stream()
+    ft.SetStream(side.cuda_stream)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        ft.FwdImageInvFFTBatch(mbuf)
+    for rep in range(2):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i, o in enumerate(outs):
+            assert np.array_equal(H.to_host(o), want_outs[i]), (rep, i)
+
+
+def test_split_bf16_fp16_io_512_templates():
+    """FP16 micrograph spectrum (__half2), FP16 512 x 512 templates and FP16 outputs, scaled as a caller
+    would (pre 1/n on the micrograph FwdFFT, post 1/(n t^2)); peaks equal the planted lags and the FP32
+    path's."""
+    from fastfft_amd import workloads as W
+
+    n, t, m = 4096, 512, 512
+    H.log("split-BF16 batch, FP16 I/O, 4096^2 x 512^2 -> 512^2")
+    torch, d = H.torch_dev()
+    mic, tmpls, lags = _planted(n, t, m, 220.0, 4, seed=31)
+    mic16 = mic.astype(np.float16)
+    tm16 = [tp.astype(np.float16) for tp in tmpls]
+    mft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1), input_type="f16", image_type="c32h")
+    mbuf = H.to_dev(W.to_padded(mic16, dtype=np.float16).reshape(-1))
+    mft.FwdFFT(mbuf, pre_op=H.F.Scale(1.0 / n))
+    tb = [H.to_dev(W.to_padded(tp, dtype=np.float16).reshape(-1)) for tp in tm16]
+    post = 1.0 / (n * t * t)
+    res = {}
+    for prec in ("split_bf16", "fp32"):
+        ft = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0), input_type="f16",
+                         image_type="c32h")
+        outs = [H.dev_zeros(ft.ReturnInvOutputMemorySize(), "f16") for _ in tb]
+        peaks = torch.full((2 * len(tb),), -1, dtype=torch.int32, device=d)
+        vals = torch.zeros(len(tb), dtype=torch.float32, device=d)
+        ft.SetPrecision(prec)
+        ft.SetBatch(tb, [(t, t)] * len(tb), outs, peaks, vals)
+        assert f"precision={prec}" in ft.Describe("batch")
+        ft.FwdImageInvFFTBatch(mbuf, post_op=H.F.Scale(post))
+        res[prec] = ([H.real_window(H.to_host(o).astype(np.float32), m, m) for o in outs],
+                     H.to_host(peaks).reshape(-1, 2))
+    spec = O.fwd_2d(W.to_padded(mic16.astype(np.float32)), n, n, n, pre_scale=1.0 / n)
+    for i, tp in enumerate(tm16):
+        want = O.xcorr_2d(W.to_padded(tp.astype(np.float32)), t, spec, n, n, m, m, -m // 2, -m // 2, post_scale=post)
+        err = O.rel_l2(res["split_bf16"][0][i], want)
+        print(f"fp16 template {i}: split-BF16 rel_l2={err:.3e} (FP32 path {O.rel_l2(res['fp32'][0][i], want):.3e})")
+        assert err < TOL_F16, i
+        assert tuple(res["split_bf16"][1][i]) == tuple(res["fp32"][1][i]), i
+        assert (res["split_bf16"][1][i][0] - m // 2, res["split_bf16"][1][i][1] - m // 2) == lags[i], i
+
+
+def test_split_bf16_leaves_other_passes_fp32():
+    """A batch whose correlation pass is not a 1024-point one runs the FP32 kernels bit for bit."""
+    from fastfft_amd import workloads as W
+
+    torch, d = H.torch_dev()
+    n, nout, origin = (256, 128), (64, 32), (-32, -16)
+    sizes = [(40, 30), (64, 64), (17, 50)]
+    rng = np.random.default_rng(41)
+    spec = (rng.standard_normal((n[0] // 2 + 1, n[1])) + 1j * rng.standard_normal((n[0] // 2 + 1, n[1]))).astype(np.complex64)
+    img = H.to_dev(H.complex_to_f32(spec))
+    tb = [H.to_dev(H.padded_real(rng.standard_normal((y, x)).astype(np.float32), x).reshape(-1)) for x, y in sizes]
+    got = {}
+    for prec in ("split_bf16", "fp32"):
+        ft = H.make_plan((64, 64, 1), (*n, 1), (*n, 1), (*nout, 1), origin=(*origin, 0))
+        outs = [H.dev_zeros(ft.ReturnInvOutputMemorySize()) for _ in sizes]
+        ft.SetPrecision(prec)
+        ft.SetBatch(tb, sizes, outs)
+        ft.FwdImageInvFFTBatch(img)
+        got[prec] = [H.to_host(o).copy() for o in outs]
+    for a, b in zip(got["split_bf16"], got["fp32"]):
+        assert np.array_equal(a, b)

@@ -197,3 +197,12 @@ def test_workspace_is_reported_without_gpu():
     # ws_a: A[384 rows][2049 kx], ws_b: B[512 rows][2049 kx] (the last row ends at kx = 2048)
     assert ft.WorkspaceBytes() == 8 * ((383 * 2050 + 2049) + (511 * 2050 + 2049))
     assert ft.GetDeviceBufferPointer() == 0  # allocated at the first transform only
+
+
+def test_set_precision_values():
+    ft = plan((64, 64, 1), (256, 256, 1), (256, 256, 1), (64, 64, 1))
+    ft.SetPrecision("split_bf16")
+    ft.SetPrecision("fp32")
+    with pytest.raises(F.FastFFTError) as e:
+        ft.SetPrecision(7)
+    assert e.value.status == F.INVALID_ARGUMENT
