@@ -85,6 +85,7 @@ def lib():
         "fastfft_set_output_window": [P, I, I, I],
         "fastfft_set_stream": [P, V],
         "fastfft_set_precision": [P, ctypes.c_int],
+        "fastfft_run_batch_stage": [P, ctypes.c_int, V, _Op, _Op, _Op],
         "fastfft_fwd": [P, V, V, _Op, _Op],
         "fastfft_inv": [P, V, V, _Op, _Op],
         "fastfft_fwd_image_inv": [P, V, V, V, _Op, _Op, _Op],
@@ -268,6 +269,13 @@ class FourierTransformer:
         self._bind_torch_stream()
         _check(lib().fastfft_fwd_image_inv_batch(self._h, _ptr(image_to_search), pre_op or Noop(), intra_op or ConjMul(),
                                                  post_op or Noop()))
+
+    def RunBatchStage(self, stage: int, image_to_search, pre_op=None, intra_op=None, post_op=None) -> None:
+        """One part of the batched call over the whole batch, for timing (see fastfft_run_batch_stage):
+        0 frame reduction, 1 R2C rows, 2 correlation pass, 3 C2R rows, 4 argmax reduction."""
+        self._bind_torch_stream()
+        _check(lib().fastfft_run_batch_stage(self._h, int(stage), _ptr(image_to_search), pre_op or Noop(),
+                                             intra_op or ConjMul(), post_op or Noop()))
 
     def Wait(self) -> None:
         _check(lib().fastfft_wait(self._h))

@@ -412,9 +412,20 @@ static fastfft_status fill_batch_items(fastfft_plan p, const std::vector<Stage>&
     return e == hipSuccess ? ok( ) : hip_fail(e, "batch descriptors");
 }
 
-// Launches every chunk of the batch: R2C rows, fused columns, C2R rows (+ argmax keys).
+// Parts of a batched call, for fastfft_run_batch_stage (timing of one part).
+enum : unsigned {
+    kBatchFrame   = 1u,  // reduction of the image to the frame (frame form only)
+    kBatchR2C     = 2u,  // R2C rows of every template
+    kBatchXcorr   = 4u,  // correlation pass
+    kBatchC2R     = 8u,  // trimmed C2R rows (+ argmax row keys)
+    kBatchArgmax  = 16u, // argmax reduction of the row keys
+    kBatchAll     = 31u
+};
+
+// Launches every chunk of the batch: R2C rows, fused columns, C2R rows (+ argmax keys);
+// parts outside `parts` are skipped.
 static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>& st, const void* image, CpxT img_ct,
-                                       float pre, float intra, float post, const float2* tw) {
+                                       float pre, float intra, float post, const float2* tw, unsigned parts = kBatchAll) {
     const int   n    = (int)p->batch.size( );
     const int   C    = p->batch_chunk;
     const bool  half = p->st.input_type == FASTFFT_INPUT_F16;
@@ -422,8 +433,10 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
     auto*       keys = reinterpret_cast<unsigned long long*>(p->batch_r2c) - (size_t)C * kArgmaxPartials;
     for ( int c0 = 0; c0 < n; c0 += C ) {
         const int  cnt = std::min(C, n - c0);
-        hipError_t e   = launch_r2c_rows_batch(st[0], p->batch_r2c + c0, cnt, st[0].nlines, rt, pre, tw, p->stream);
-        if ( e == hipSuccess ) {
+        hipError_t e   = hipSuccess;
+        if ( parts & kBatchR2C )
+            e = launch_r2c_rows_batch(st[0], p->batch_r2c + c0, cnt, st[0].nlines, rt, pre, tw, p->stream);
+        if ( e == hipSuccess && (parts & kBatchXcorr) ) {
             StageIO io;
             io.img     = image;
             io.img_ct  = img_ct;
@@ -433,9 +446,9 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
             io.split_bf16 = p->precision == FASTFFT_PRECISION_SPLIT_BF16;
             e          = launch_xcorr(st[1], io, tw, p->stream);
         }
-        if ( e == hipSuccess )
+        if ( e == hipSuccess && (parts & kBatchC2R) )
             e = launch_c2r_rows_batch(st[2], p->batch_c2r + c0, cnt, rt, post, tw, p->stream);
-        if ( e == hipSuccess && p->batch_peaks )
+        if ( e == hipSuccess && p->batch_peaks && (parts & kBatchArgmax) )
             e = launch_argmax_keys(keys, st[2].nlines, st[2].M, p->batch_values + c0, p->batch_peaks + 2 * c0, p->stream, cnt,
                                    kArgmaxPartials);
         if ( e != hipSuccess )
@@ -657,15 +670,17 @@ static fastfft_status setup_subframe(fastfft_plan p) {
 
 // The batch on the sub-frame: image -> frame -> frame spectrum, then every chunk.
 static fastfft_status run_subframe_batch(fastfft_plan p, const void* image, float pre, float intra, float post,
-                                         const float2* tw) {
+                                         const float2* tw, unsigned parts = kBatchAll) {
     const float frame_scale = 1.f / ((float)p->sub_fx * (float)p->sub_fy);
-    if ( auto rc = run_stage_list(p, p->sub_inv, "batch frame", const_cast<void*>(image), p->sub_s, nullptr, 1.f, 1.f,
-                                  frame_scale, 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
-        return rc;
-    if ( auto rc = run_stage_list(p, p->sub_fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f, 0, -1,
-                                  nullptr, nullptr, p->sub_wa, p->sub_wb) )
-        return rc;
-    return run_batch_chunks(p, p->sub_xc, p->sub_h, CpxT::c64, pre, intra, post, tw);
+    if ( parts & kBatchFrame ) {
+        if ( auto rc = run_stage_list(p, p->sub_inv, "batch frame", const_cast<void*>(image), p->sub_s, nullptr, 1.f, 1.f,
+                                      frame_scale, 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+            return rc;
+        if ( auto rc = run_stage_list(p, p->sub_fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f, 0,
+                                      -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+            return rc;
+    }
+    return run_batch_chunks(p, p->sub_xc, p->sub_h, CpxT::c64, pre, intra, post, tw, parts);
 }
 
 extern "C" {
@@ -1002,6 +1017,30 @@ fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_
     if ( ! plan->batch.empty( ) )
         plan->stage_completed = 7;
     return ok( );
+}
+
+fastfft_status fastfft_run_batch_stage(fastfft_plan plan, int stage, const void* image_to_search, fastfft_op pre_op,
+                                       fastfft_op intra_op, fastfft_op post_op) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( stage < 0 || stage > 4 || ! image_to_search )
+        return fail(FASTFFT_INVALID_ARGUMENT, "run_batch_stage: stage must be 0..4 and the image not null");
+    if ( plan->batch_chunk <= 0 || reinterpret_cast<std::uintptr_t>(image_to_search) % 16 != 0 )
+        return fail(FASTFFT_UNSUPPORTED, "run_batch_stage: the batch runs per template (no batched kernels)");
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(plan, 2, &sp) )
+        return rc;
+    float2* tw = nullptr;
+    if ( auto rc = ensure_device_state(plan, &tw) )
+        return rc;
+    const unsigned part = 1u << stage;
+    const CpxT     ct   = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+    if ( plan->sub_active )
+        return run_subframe_batch(plan, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), tw, part);
+    if ( part == kBatchFrame )
+        return ok( );
+    return run_batch_chunks(plan, *sp, image_to_search, ct, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), tw,
+                            part);
 }
 
 fastfft_status fastfft_wait(fastfft_plan plan) {

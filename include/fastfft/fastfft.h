@@ -194,6 +194,18 @@ fastfft_status fastfft_argmax_device(fastfft_plan plan, const void* img, int nx,
 fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, int last, void* in, const void* image_to_search,
                                   void* out, fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op);
 
+/* Runs one part of the batched FwdImageInvFFTBatch over the whole batch, for
+ * per-kernel timing (no counterpart in the reference): 0 the reduction of the
+ * image to the batch's frame (nothing when the batch runs on the full frame),
+ * 1 the R2C rows, 2 the correlation pass, 3 the trimmed C2R rows (with the
+ * argmax row keys), 4 the argmax reduction.  A part reads what the previous
+ * part last left in the batch's buffers, which hold one chunk of templates:
+ * the parts run in order give the call's outputs only for a batch of one
+ * chunk (fastfft_describe "batch" shows the chunk).  FASTFFT_UNSUPPORTED when
+ * the batch runs template by template. */
+fastfft_status fastfft_run_batch_stage(fastfft_plan plan, int stage, const void* image_to_search, fastfft_op pre_op,
+                                       fastfft_op intra_op, fastfft_op post_op);
+
 /* Human-readable description of the stage sequence an operation would run
  * ("fwd", "inv" or "xcorr"), written without touching the GPU; "batch"
  * describes the batch of fastfft_set_batch (its chunk size, and the frame and

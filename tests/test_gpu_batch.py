@@ -127,3 +127,27 @@ def test_batch_rejects_oversized_template():
     with pytest.raises(H.F.FastFFTError) as e:
         ft.SetBatch([t], [(128, 128)], [o])
     assert e.value.status == H.F.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_batch_parts_in_order_equal_the_call(case, monkeypatch):
+    """fastfft_run_batch_stage 0..4 in order (frame, R2C rows, correlation pass, C2R rows, argmax) gives
+    the outputs and peaks of one FwdImageInvFFTBatch call, bit for bit, for a batch of one chunk."""
+    monkeypatch.setenv("FASTFFT_BATCH_CHUNK", "8")
+    sizes, n, nout, origin = BATCHES[case]
+    H.log(f"batch parts in order, {sizes} in {n} -> {nout}")
+    torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=13)
+    assert "chunk=8" in ft.Describe("batch")
+    ft.FwdImageInvFFTBatch(img)
+    torch.cuda.synchronize()
+    want = [H.to_host(o).copy() for o in outs]
+    want_pk = H.to_host(peaks).copy()
+    for o in outs:
+        o.zero_()
+    peaks.fill_(-1)
+    for part in range(5):
+        ft.RunBatchStage(part, img)
+    torch.cuda.synchronize()
+    for i, o in enumerate(outs):
+        assert np.array_equal(H.to_host(o), want[i]), i
+    assert np.array_equal(H.to_host(peaks), want_pk)
