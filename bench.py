@@ -17,6 +17,8 @@ B correlations x steps / time.
 Beside the headline line the JSON holds:
   split_bf16  the same batch with the correlation pass on the matrix cores
               (fastfft_set_precision), its speedup over FP32 and its parity;
+  uneven_batch 256 templates of mixed sizes (96 x 128^2, 96 x 256^2,
+              64 x 384^2) in one call per step;
   single      one correlation at a time (fastfft_fwd_image_inv_argmax, the
               FwdImageInvFFT pipeline on the full 4096 x 4096 frame);
   roofline    the dominant kernel of a step (the batched correlation pass):
@@ -180,6 +182,31 @@ def main() -> int:
     s_peaks_same = bool(np.array_equal(spk.cpu().numpy(), b_pk_all))
     sft.SetStream(torch.cuda.current_stream())
 
+    # ---------------------------------------------------------------- uneven batch of 256 (96 x 128^2,
+    # 96 x 256^2, 64 x 384^2), one call per step
+    utm = W.batched_templates()
+    usizes = [(t.shape[1], t.shape[0]) for t in utm]
+    ubufs = [torch.from_numpy(W.to_padded(t).reshape(-1)).to(dev) for t in utm]
+    uft = plan()
+    uouts = [torch.zeros(uft.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev) for _ in utm]
+    upk = torch.zeros(2 * len(utm), dtype=torch.int32, device=dev)
+    uval = torch.zeros(len(utm), dtype=torch.float32, device=dev)
+    uft.SetBatch(ubufs, usizes, uouts, upk, uval)
+    uft.SetStream(cap)
+    g_u = capture(lambda: uft.FwdImageInvFFTBatch(spec, intra_op=conj), 1)
+    u_reps = max(2, K // 4)
+    u_ms = time_graph(g_u, u_reps, warm=3) / u_reps
+    uft.SetStream(torch.cuda.current_stream())
+    uneven = {
+        "templates": len(utm),
+        "sizes": "96 x 128^2, 96 x 256^2, 64 x 384^2 (circular masks), 512^2 window",
+        "value": round(len(utm) * 1000.0 / u_ms, 1),
+        "ms_per_call": round(u_ms, 5),
+        "form": uft.Describe("batch").splitlines()[0],
+        "timing": f"HIP graph of one FwdImageInvFFTBatch call replayed {u_reps} times",
+    }
+    del uouts, ubufs
+
     # ---------------------------------------------------------------- single correlations
     ft = plan()
     stages = ft.Describe("xcorr").splitlines()[1:]
@@ -270,6 +297,7 @@ def main() -> int:
             "speedup_vs_fp32": round(ms_step / s_ms, 3),
             "peaks_identical_to_fp32": s_peaks_same,
         },
+        "uneven_batch": uneven,
         "single": {
             "value": round(1000.0 / single_ms, 1),
             "ms_per_corr": round(single_ms, 5),

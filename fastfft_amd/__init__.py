@@ -75,6 +75,14 @@ def lib():
     if not os.path.exists(path):
         raise FastFFTError(HIP_ERROR, f"{path} is not built; run `python fastfft_amd/build.py` "
                                       "(or __graft_entry__.build()) first")
+    # A process that also uses torch must end up with torch's HIP runtime (torch ships its
+    # own libamdhip64 / libhsa-runtime64): with this library's /opt/rocm runtime loaded
+    # first, the library's HIP calls found no device once torch had initialised.  Loading
+    # torch first makes both bind the same runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     P, I, V, S, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float
     sig = {

@@ -374,8 +374,15 @@ static fastfft_status argmax_scratch(fastfft_plan plan, ArgmaxScratch& a) {
 static std::mutex                 g_chunk_mutex;
 static std::map<std::string, int> g_chunk_cache;
 
-static size_t slot_elems_a(const Stage& r2c) { return (size_t)r2c.nlines * (size_t)r2c.om.s1; }
-static size_t slot_elems_b(const Stage& c2r) { return (size_t)c2r.nlines * (size_t)c2r.im.s1; }
+// Elements a line map addresses for nlines lines of nelem elements (single-index lines).
+static size_t map_extent(const HostLineMap& m, int nlines, int nelem) {
+    return (size_t)((long)(nlines - 1) * m.s1 + (long)(nelem - 1) * m.es + 1);
+}
+// A slot: the R2C rows' output (nlines template rows of nout values); B slot: the
+// correlation pass's output as the C2R pass reads it (nlines rows of the spectrum's
+// w values), rounded to even counts so that every slot starts 16-byte aligned.
+static size_t slot_elems_a(const Stage& r2c) { return (map_extent(r2c.om, r2c.nlines, r2c.nout) + 1) / 2 * 2; }
+static size_t slot_elems_b(const Stage& c2r) { return (map_extent(c2r.im, c2r.nlines, c2r.N / 2 + 1) + 1) / 2 * 2; }
 
 // Device block for chunks of C templates: A slots, B slots, argmax keys, then the
 // descriptor arrays of all n templates (filled by fill_batch_items).
@@ -624,8 +631,9 @@ static fastfft_status setup_subframe(fastfft_plan p) {
     fill_sizes(ps);
     ps.validated = false;
     std::vector<Stage> xc, inv, fwd;
-    if ( validate(ps).code || build_stages(ps, "xcorr", xc).code || xc.size( ) != 3 ||
-         ! batch_kernels_cover(xc[0], xc[1], xc[2], CpxT::c64) )
+    if ( validate(ps).code || build_stages(ps, "xcorr", xc).code || xc.size( ) != 3 )
+        return ok( );
+    if ( ! batch_kernels_cover(xc[0], xc[1], xc[2], CpxT::c64) )
         return ok( );
     // the image reduced to the frame: trimmed inverse of the N-frame spectrum (read in the plan's image type) ...
     PlanState pi = frame_plan(FASTFFT_INPUT_F32, nn, Dims{fx, fy, 1, 0}, st.origin);

@@ -49,6 +49,11 @@ struct StageIO {
 bool       r2c_rows_batch_covers(const Stage& s);
 bool       c2r_rows_batch_covers(const Stage& s);
 bool       xcorr_col_covers(const Stage& s, CpxT img_ct);
+// complex forward / inverse line stages over adjacent lines with far-apart
+// elements (3-D z-passes), N = 256 or 512: tiled kernel (tile_lines.hpp);
+// hipErrorNotSupported, before launching anything, when it does not cover s
+bool       tile_lines_cover(const Stage& s);
+hipError_t launch_tile_lines(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 // the split-BF16 matrix-core correlation pass (xcorr_mfma.hpp): N = 1024, L <= 512,
 // window origin 0, single-index lines with unit line stride, FP32 image
 bool       xcorr_mfma_covers(const Stage& s, CpxT img_ct);

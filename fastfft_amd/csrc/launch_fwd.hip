@@ -155,6 +155,11 @@ hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipSt
         if ( e != hipErrorNotSupported )
             return e;
     }
+    if ( s.kind == StageKind::fwd_cpx && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = launch_tile_lines(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && rows_ok && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = io.src_rt == RealT::f32 ? try_r2c_wave<float>(s, io, tw, st) : try_r2c_wave<__half>(s, io, tw, st);
         if ( e != hipErrorNotSupported )

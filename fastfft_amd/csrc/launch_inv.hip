@@ -227,6 +227,11 @@ hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipSt
         if ( e != hipErrorNotSupported )
             return e;
     }
+    if ( s.kind == StageKind::inv_cpx && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = launch_tile_lines(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     // (an argmax asked of this C2R pass stays unfused: the caller runs it separately)
     if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = try_window(s, io, tw, st);

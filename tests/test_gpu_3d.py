@@ -160,3 +160,37 @@ def test_full_inv_512_to_128_sampled_and_cropped():
         want += np.einsum("zs,zs->s", u, ez[z0:z0 + 32])
     gv = got[pts[:, 2], pts[:, 1], pts[:, 0]]
     assert O.rel_l2(gv, want.real) < TOL
+
+
+# ------------------------------------------------------------------ tiled z-passes (N = 256, 512)
+@pytest.mark.parametrize("mode", ["default", "no_tile"])
+@pytest.mark.parametrize("vin,n", [((20, 30, 40), (64, 64, 512)), ((32, 32, 32), (64, 64, 256))])
+def test_fwd_3d_tiled_z_pass(vin, n, mode, monkeypatch):
+    """Forward transforms whose z-pass (adjacent lines, far-apart elements) runs as tiles of 16 lines,
+    against the FP64 oracle, with and without the tiled kernel."""
+    if mode == "no_tile":
+        monkeypatch.setenv("FASTFFT_NO_TILE_LINES", "1")
+    H.log(f"3d fwd {vin} -> {n}, z-pass mode={mode}")
+    v = _vol(vin[::-1], seed=sum(vin) + 1)
+    got = _fwd_gpu(v, n)
+    want = O.fwd_3d(H.padded_real(v, vin[0]), vin[0], *n)
+    assert O.rel_l2(got, want) < TOL
+
+
+@pytest.mark.parametrize("mode", ["default", "no_tile"])
+def test_inv_3d_tiled_z_pass_window(mode, monkeypatch):
+    """A trimmed inverse whose z-pass keeps 100 of 512 lags at an origin off the 64-lane grid."""
+    if mode == "no_tile":
+        monkeypatch.setenv("FASTFFT_NO_TILE_LINES", "1")
+    n, nout, origin = (128, 64, 512), (100, 60, 100), (-50, -30, -37)
+    H.log(f"3d inv {n} -> {nout} at {origin}, z-pass mode={mode}")
+    rng = np.random.default_rng(77)
+    w = n[0] // 2 + 1
+    spec = (rng.standard_normal((n[2], w, n[1])) + 1j * rng.standard_normal((n[2], w, n[1]))).astype(np.complex64)
+    ft = H.make_plan(n, n, n, nout, origin=origin, rank=3)
+    a = H.to_dev(H.complex_to_f32(spec))
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    ft.InvFFT(a, out)
+    got = H.real_window(H.to_host(out), *nout)
+    want = O.inv_3d(spec, *n, *nout, *origin)
+    assert O.rel_l2(got, want) < TOL

@@ -151,3 +151,45 @@ def test_batch_parts_in_order_equal_the_call(case, monkeypatch):
     for i, o in enumerate(outs):
         assert np.array_equal(H.to_host(o), want[i]), i
     assert np.array_equal(H.to_host(peaks), want_pk)
+
+
+def test_batch_256_uneven_templates_4096():
+    """The full uneven batch: 96 templates of 128^2, 96 of 256^2 and 64 of 384^2 (circular masks) against one
+    4096^2 spectrum, 512^2 centred window.  Every peak equals the peak of the template's single correlation,
+    and every 16th output equals its single correlation (rel-L2 within 2e-6)."""
+    from fastfft_amd import workloads as W
+
+    H.log("batch of 256 uneven templates against 4096^2")
+    torch, d = H.torch_dev()
+    n, m = 4096, 512
+    mic, _, _ = W.xcorr_config()
+    mft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1))
+    img = H.dev_zeros(mft.ReturnFwdOutputMemorySize())
+    img[:] = H.to_dev(W.to_padded(mic).reshape(-1))
+    mft.FwdFFT(img)
+    tmpls = W.batched_templates()
+    sizes = [(t.shape[1], t.shape[0]) for t in tmpls]
+    ft = H.make_plan((384, 384, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
+    tb = [H.to_dev(W.to_padded(t).reshape(-1)) for t in tmpls]
+    outs = [H.dev_zeros(ft.ReturnInvOutputMemorySize()) for _ in tmpls]
+    peaks = torch.full((2 * len(tmpls),), -1, dtype=torch.int32, device=d)
+    vals = torch.zeros(len(tmpls), dtype=torch.float32, device=d)
+    ft.SetBatch(tb, sizes, outs, peaks, vals)
+    assert "frame=1024x1024" in ft.Describe("batch")
+    ft.FwdImageInvFFTBatch(img)
+    pk = H.to_host(peaks).reshape(-1, 2)
+    singles = {}
+    for i, (x, y) in enumerate(sizes):
+        if (x, y) not in singles:
+            singles[(x, y)] = H.make_plan((x, y, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
+        s = singles[(x, y)]
+        a = H.dev_zeros(max(s.ReturnInputMemorySize(), s.ReturnInvOutputMemorySize()))
+        a[: tb[i].numel()] = tb[i]
+        o = H.dev_zeros(s.ReturnInvOutputMemorySize())
+        sp = torch.zeros(2, dtype=torch.int32, device=d)
+        sv = torch.zeros(1, dtype=torch.float32, device=d)
+        s.FwdImageInvFFTArgMax(a, img, o, sp, sv)
+        assert tuple(pk[i]) == tuple(H.to_host(sp)), i
+        if i % 16 == 0:
+            got = H.real_window(H.to_host(outs[i]), m, m)
+            assert O.rel_l2(got, H.real_window(H.to_host(o), m, m)) < TOL, i
