@@ -19,6 +19,8 @@ Beside the headline line the JSON holds:
               (fastfft_set_precision), its speedup over FP32 and its parity;
   uneven_batch 256 templates of mixed sizes (96 x 128^2, 96 x 256^2,
               64 x 384^2) in one call per step;
+  config4_3d  the 3-D size changes: padded forward 64^3 -> 256^3 and trimmed
+              inverse 512^3 -> 128^3 (centred), with their HBM fractions;
   single      one correlation at a time (fastfft_fwd_image_inv_argmax, the
               FwdImageInvFFT pipeline on the full 4096 x 4096 frame);
   roofline    the dominant kernel of a step (the batched correlation pass):
@@ -207,6 +209,50 @@ def main() -> int:
     }
     del uouts, ubufs
 
+    # ---------------------------------------------------------------- 3-D size change: padded forward
+    # 64^3 -> 256^3 and trimmed inverse 512^3 -> 128^3 (centred), each timed alone
+    def time_3d(setup):
+        p, fn = setup()
+        p.SetStream(cap)
+        with torch.cuda.stream(cap):
+            fn()  # first call: workspace and twiddle table, outside the capture
+        torch.cuda.synchronize()
+        g3 = capture(fn, 1)
+        r3 = max(2, K // 20)
+        ms3 = time_graph(g3, r3) / r3
+        p.SetStream(torch.cuda.current_stream())
+        return p, ms3
+
+    def fwd3():
+        p = F.FourierTransformer("f32", "c64", 3)
+        p.SetForwardFFTPlan(64, 64, 64, 256, 256, 256)
+        p.SetInverseFFTPlan(256, 256, 256, 256, 256, 256)
+        vin = torch.from_numpy(W.to_padded(np.random.default_rng(6).standard_normal((64, 64, 64)).astype(np.float32))
+                               .reshape(-1)).to(dev)
+        vout = torch.zeros(p.ReturnFwdOutputMemorySize(), dtype=torch.float32, device=dev)
+        return p, lambda: p.FwdFFT(vin, vout)
+
+    def inv3():
+        p = F.FourierTransformer("f32", "c64", 3)
+        p.SetForwardFFTPlan(512, 512, 512, 512, 512, 512)
+        p.SetInverseFFTPlan(512, 512, 512, 128, 128, 128)
+        p.SetOutputWindow(-64, -64, -64)
+        sp3 = torch.randn(512 * 257 * 512 * 2, dtype=torch.float32, device=dev)
+        o3 = torch.zeros(p.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev)
+        return p, lambda: p.InvFFT(sp3, o3)
+
+    _, f3_ms = time_3d(fwd3)
+    _, i3_ms = time_3d(inv3)
+    f3_bytes = 64 * 64 * 66 * 4 + 256 * 129 * 256 * 8
+    i3_bytes = 512 * 257 * 512 * 8 + 128 * 128 * 130 * 4
+    config4 = {
+        "fwd_64_to_256": {"ms": round(f3_ms, 5), "compulsory_bytes": f3_bytes,
+                          "hbm_frac": round(f3_bytes / (f3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "inv_512_to_128_centred": {"ms": round(i3_ms, 5), "compulsory_bytes": i3_bytes,
+                                   "hbm_frac": round(i3_bytes / (i3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "timing": "HIP graph of one transform, replayed; parity in tests/test_gpu_3d.py",
+    }
+
     # ---------------------------------------------------------------- single correlations
     ft = plan()
     stages = ft.Describe("xcorr").splitlines()[1:]
@@ -298,6 +344,7 @@ def main() -> int:
             "peaks_identical_to_fp32": s_peaks_same,
         },
         "uneven_batch": uneven,
+        "config4_3d": config4,
         "single": {
             "value": round(1000.0 / single_ms, 1),
             "ms_per_corr": round(single_ms, 5),

@@ -221,14 +221,15 @@ hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipSt
         if ( e != hipErrorNotSupported )
             return e;
     }
-    if ( s.kind == StageKind::inv_c2r && io.dst_real && ! io.src_real && io.src_ct == CpxT::c64 &&
-         ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
-        hipError_t e = io.dst_rt == RealT::f32 ? try_c2r_wave<float>(s, io, tw, st) : try_c2r_wave<__half>(s, io, tw, st);
+    // 256 / 512-point lines, 16 per workgroup (a C2R pass asked for its argmax keeps the row kernel)
+    if ( (s.kind == StageKind::inv_cpx || (s.kind == StageKind::inv_c2r && ! io.amax)) && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = launch_tile_lines(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;
     }
-    if ( s.kind == StageKind::inv_cpx && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
-        hipError_t e = launch_tile_lines(s, io, tw, st);
+    if ( s.kind == StageKind::inv_c2r && io.dst_real && ! io.src_real && io.src_ct == CpxT::c64 &&
+         ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = io.dst_rt == RealT::f32 ? try_c2r_wave<float>(s, io, tw, st) : try_c2r_wave<__half>(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;
     }

@@ -1,40 +1,64 @@
-// Tiled z-pass line transforms (tile_lines.hpp) for the strided passes of the
-// 3-D transforms.
+// Line transforms of 256 or 512 points, 16 lines per workgroup (tile_lines.hpp),
+// for the passes of the 3-D transforms.
 #include "launch_common.hpp"
 #include "tile_lines.hpp"
 
 namespace fastfft_amd {
 
-template <int N, int DIR, typename TIn>
+template <int N, int DIR, typename TIn, bool HERM, typename TOut, bool TI, bool TO>
 static hipError_t go_tile(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st, int M, int o) {
-    k_tile_lines<N, DIR, TIn><<<dim3((unsigned)(s.nlines / kTileLines)), dim3(512), 0, st>>>(
-            static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<float2*>(io.dst), to_device(s.om), M, o,
+    k_tile_lines<N, DIR, TIn, HERM, TOut, TI, TO><<<dim3((unsigned)(s.nlines / kTileLines)), dim3(512), 0, st>>>(
+            static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<TOut*>(io.dst), to_device(s.om), M, o,
             s.nlines, io.scale, tw);
     return hipGetLastError( );
 }
 
+// lines of a tile next to each other (unit stride between lines, inside one
+// group of a two-index map) with far-apart elements
+static bool tile_map(const HostLineMap& m) {
+    const bool adjacent = (m.n2 == 1 && m.s1 == 1) || (m.n2 > 1 && m.s2 == 1 && m.n2 % kTileLines == 0);
+    return adjacent && m.es >= kTileLines;
+}
+
+static bool row_map(const HostLineMap& m) { return m.es == 1; }
+
 bool tile_lines_cover(const Stage& s) {
-    // adjacent lines (unit line stride), far-apart elements, whole tiles, N <= 512 (one wave per transform)
-    return (s.N == 256 || s.N == 512) && s.im.n2 == 1 && s.om.n2 == 1 && s.im.s1 == 1 && s.om.s1 == 1 &&
-           s.im.es >= kTileLines && s.om.es >= kTileLines && s.nlines % kTileLines == 0 && s.nlines > 0;
+    if ( ! (s.N == 256 || s.N == 512) || s.nlines <= 0 || s.nlines % kTileLines != 0 )
+        return false;
+    if ( s.kind == StageKind::inv_c2r )
+        return tile_map(s.im) && row_map(s.om);
+    if ( s.kind != StageKind::fwd_cpx && s.kind != StageKind::inv_cpx )
+        return false;
+    return (tile_map(s.im) && tile_map(s.om)) || (row_map(s.im) && row_map(s.om));
 }
 
 hipError_t launch_tile_lines(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( env_int("FASTFFT_NO_TILE_LINES", 0) || ! tile_lines_cover(s) )
         return hipErrorNotSupported;
-    const bool fwd = s.kind == StageKind::fwd_cpx, inv = s.kind == StageKind::inv_cpx;
-    if ( ! (fwd || inv) || io.src_real || io.dst_real || io.dst_ct != CpxT::c64 )
-        return hipErrorNotSupported;
-    const int M = fwd ? s.nout : s.M, o = fwd ? 0 : s.o;
-    if ( M < 1 || M > s.N )
+    const bool fwd = s.kind == StageKind::fwd_cpx, c2r = s.kind == StageKind::inv_c2r;
+    const int  M = fwd ? s.nout : s.M, o = fwd ? 0 : s.o;
+    if ( M < 1 || M > s.N || io.src_real )
         return hipErrorNotSupported;
     auto go = [&](auto nc) -> hipError_t {
         constexpr int N = decltype(nc)::value;
-        if ( fwd )
-            return io.src_ct == CpxT::c64 ? go_tile<N, -1, float2>(s, io, tw, st, M, o)
-                                          : go_tile<N, -1, __half2>(s, io, tw, st, M, o);
-        return io.src_ct == CpxT::c64 ? go_tile<N, +1, float2>(s, io, tw, st, M, o)
-                                      : go_tile<N, +1, __half2>(s, io, tw, st, M, o);
+        if ( c2r ) {
+            if ( ! io.dst_real || io.src_ct != CpxT::c64 )
+                return hipErrorNotSupported;
+            return io.dst_rt == RealT::f32 ? go_tile<N, +1, float2, true, float, true, false>(s, io, tw, st, M, o)
+                                           : go_tile<N, +1, float2, true, __half, true, false>(s, io, tw, st, M, o);
+        }
+        if ( io.dst_real || io.dst_ct != CpxT::c64 )
+            return hipErrorNotSupported;
+        const bool tiled = tile_map(s.im) && tile_map(s.om);
+        auto       dir   = [&](auto dc) -> hipError_t {
+            constexpr int D = decltype(dc)::value;
+            if ( tiled )
+                return io.src_ct == CpxT::c64 ? go_tile<N, D, float2, false, float2, true, true>(s, io, tw, st, M, o)
+                                              : go_tile<N, D, __half2, false, float2, true, true>(s, io, tw, st, M, o);
+            return io.src_ct == CpxT::c64 ? go_tile<N, D, float2, false, float2, false, false>(s, io, tw, st, M, o)
+                                          : go_tile<N, D, __half2, false, float2, false, false>(s, io, tw, st, M, o);
+        };
+        return fwd ? dir(std::integral_constant<int, -1>{ }) : dir(std::integral_constant<int, 1>{ });
     };
     return s.N == 256 ? go(std::integral_constant<int, 256>{ }) : go(std::integral_constant<int, 512>{ });
 }

@@ -1,19 +1,28 @@
-// Complex line transforms of N <= 512 points whose lines lie next to each other
-// in memory and whose elements are far apart (the z-passes of the 3-D
-// transforms: C[kz][kx][ky] lines along kz), the reference's strided 3-D
-// kernels (block_fft_kernel_C2C_INCREASE / C2C_NONE_XYZ / C2C_DECREASE with the
-// XZ/XYZ permutes, src/fastfft/FastFFT.cu and include/detail/memory_addressing.h:25-104).
+// Line transforms of N <= 512 points, 16 lines per workgroup: the passes of the
+// 3-D transforms (the reference's strided 3-D kernels, block_fft_kernel_
+// C2C_INCREASE / C2C_NONE_XYZ / C2C_DECREASE / C2R_DECREASE with the XZ/XYZ
+// permutes, src/fastfft/FastFFT.cu and include/detail/memory_addressing.h:25-104).
 //
-// A workgroup of 8 wavefronts takes kTileLines adjacent lines: it loads element
-// rows of the tile (kTileLines consecutive values, one 128-byte access per row;
-// all of a thread's loads are issued before its LDS writes) into LDS, each wave
-// transforms two lines from LDS in registers (one P = N point Stockham
-// transform, so no sub-transform split; the line's own LDS row is its exchange
-// buffer), writes the kept window of each line back into that row, and the
-// workgroup stores the window rows the same coalesced way.
+// A workgroup of 8 wavefronts takes kTileLines lines; each wave transforms two
+// of them in registers (one P = N point Stockham transform, no sub-transform
+// split), so the twiddle set-up is shared by 16 lines.
+//   TILE_IN:  the lines lie next to each other and their elements far apart (a
+//             z-pass): element rows of the tile (16 consecutive values, one
+//             128-byte access) are loaded into LDS, all of a thread's loads
+//             issued before its LDS writes; else each wave reads its line
+//             directly (unit-stride lines: coalesced).
+//   TILE_OUT: likewise for the output: the kept window of each line goes back
+//             into its LDS row and the workgroup stores window rows; else each
+//             wave stores its own line directly.
+//   HERM:     the input line holds X[0 .. N/2] of a Hermitian sequence (X[N-k] =
+//             conj X[k] above) and only the real part is stored (C2R).
 //   forward (DIR = -1): X[k] = sum_{n < L} x[n] W_N^{n k}, k < M (window origin 0)
 //   inverse (DIR = +1): y[j] = sum_k X[k] W_N^{-k (o + j)}, j < M
+// A line's LDS row also serves as the exchange buffer of its transform once
+// the line's values are in registers.
 #pragma once
+
+#include <type_traits>
 
 #include "rows_wave.hpp"
 
@@ -21,15 +30,17 @@ namespace fastfft_amd {
 
 constexpr int kTileLines = 16;
 
-// Grid: nlines / kTileLines workgroups of 512 threads.  im.s1 = om.s1 = 1 (line
-// l + 1 follows line l), element strides im.es and om.es.
-template <int N, int DIR, typename TIn>
+// Grid: nlines / kTileLines workgroups of 512 threads.  TILE_IN / TILE_OUT need
+// lines l .. l + 15 of a tile at base(l) + 0 .. 15 (unit stride between lines).
+template <int N, int DIR, typename TIn, bool HERM, typename TOut, bool TILE_IN, bool TILE_OUT>
 __global__ void __launch_bounds__(512)
-        k_tile_lines(const TIn* __restrict__ in, LineMap im, int L, float2* __restrict__ out, LineMap om, int M, int o,
+        k_tile_lines(const TIn* __restrict__ in, LineMap im, int L, TOut* __restrict__ out, LineMap om, int M, int o,
                      int nlines, float scale, const float2* __restrict__ tw) {
-    constexpr int T = 64, E = N / T, LT = kTileLines, NW = 8, PITCH = N + 2, RPT = N / (512 / LT);
-    using FFT       = WaveFFT<N>;
+    constexpr int  T = 64, E = N / T, LT = kTileLines, NW = 8, PITCH = N + 2, RPT = N / (512 / LT), H = N / 2;
+    constexpr bool REAL_OUT = ! std::is_same<TOut, float2>::value;
+    using FFT               = WaveFFT<N>;
     static_assert(FFT::SLOTS <= PITCH, "a tile row must hold the exchange buffer of its transform");
+    static_assert(! (REAL_OUT && TILE_OUT), "real outputs are stored by row");
     __shared__ float2 s_tile[LT * PITCH];
     __shared__ float2 s_twp[N];
     const int     w     = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
@@ -37,19 +48,19 @@ __global__ void __launch_bounds__(512)
     const long    line0 = (long)blockIdx.x * LT;
     const float2* twm   = multi_twiddles(tw);
     FFT::fill_pass_table(s_twp, twm, threadIdx.x, 512);
-    // ---- element rows of the tile: thread (r, l) loads element r + 32 i of line l
-    const int  kin = DIR < 0 ? (L < N ? L : N) : N;
-    const auto* ip = as_global(in) + im.base(line0);
-    {
-        const int l = threadIdx.x % LT, r = threadIdx.x / LT; // r < 32
-        float2    ld[RPT];
+    // values stored per line: N/2 + 1 of a Hermitian line, the non-zero inputs of a forward, else N
+    const int kin = HERM ? H + 1 : (DIR < 0 ? (L < N ? L : N) : N);
+    if constexpr ( TILE_IN ) {
+        const auto* ip = as_global(in) + im.base(line0);
+        const int   l = threadIdx.x % LT, r = threadIdx.x / LT; // r < 32
+        float2      ld[RPT + 1];                                // RPT + 1 rows: N / 2 + 1 values of a Hermitian line
 #pragma unroll
-        for ( int i = 0; i < RPT; i++ ) {
+        for ( int i = 0; i < RPT + 1; i++ ) {
             const int k = r + (512 / LT) * i;
             ld[i]       = k < kin ? load_cpx(ip, (long)k * im.es + l) : make_float2(0.f, 0.f);
         }
 #pragma unroll
-        for ( int i = 0; i < RPT; i++ ) {
+        for ( int i = 0; i < RPT + 1; i++ ) {
             const int k = r + (512 / LT) * i;
             if ( k < kin )
                 s_tile[l * PITCH + k] = ld[i];
@@ -61,27 +72,66 @@ __global__ void __launch_bounds__(512)
     const int oN = ((o % N) + N) % N;
 #pragma unroll 1
     for ( int h = 0; h < LT / NW; h++ ) {
-        const int l  = w + NW * h;
+        const int l   = w + NW * h;
         float2*   row = s_tile + l * PITCH;
         float2    v[E];
+        if constexpr ( TILE_IN ) {
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int n = t + T * i;
-            v[i]        = n < kin ? row[n] : make_float2(0.f, 0.f);
+            for ( int i = 0; i < E; i++ ) {
+                const int n = t + T * i;
+                if constexpr ( HERM ) {
+                    const bool   lo = n <= H;
+                    const float2 z  = row[lo ? n : N - n];
+                    v[i]            = lo ? z : make_float2(z.x, -z.y);
+                }
+                else {
+                    v[i] = n < kin ? row[n] : make_float2(0.f, 0.f);
+                }
+            }
+            wave_lds_fence( ); // every lane has read the row before it becomes the exchange buffer
         }
-        fft.template run<DIR>(v, row, t); // the row's values are in registers: it is free
-        // value n of the transform is output j = (n - o) mod N, kept when j < M
+        else {
+            const auto* lp = as_global(in) + im.base(line0 + l);
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            const int j = (t + T * i - oN + N) & (N - 1);
-            if ( j < M )
-                row[j] = make_float2(v[i].x * scale, v[i].y * scale);
+            for ( int i = 0; i < E; i++ ) {
+                const int n = t + T * i;
+                if constexpr ( HERM ) {
+                    const bool   lo = n <= H;
+                    const float2 z  = load_cpx(lp, (long)(lo ? n : N - n) * im.es);
+                    v[i]            = lo ? z : make_float2(z.x, -z.y);
+                }
+                else {
+                    v[i] = n < kin ? load_cpx(lp, (long)n * im.es) : make_float2(0.f, 0.f);
+                }
+            }
+        }
+        fft.template run<DIR>(v, row, t);
+        // value n of the transform is output j = (n - o) mod N, kept when j < M
+        if constexpr ( TILE_OUT ) {
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                const int j = (t + T * i - oN + N) & (N - 1);
+                if ( j < M )
+                    row[j] = make_float2(v[i].x * scale, v[i].y * scale);
+            }
+        }
+        else {
+            auto* op = as_global(out) + om.base(line0 + l);
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                const int j = (t + T * i - oN + N) & (N - 1);
+                if ( j < M ) {
+                    if constexpr ( REAL_OUT )
+                        store_real(op, (long)j * om.es, v[i].x * scale);
+                    else
+                        store_cpx(op, (long)j * om.es, make_float2(v[i].x * scale, v[i].y * scale));
+                }
+            }
         }
     }
-    __syncthreads( );
-    // ---- window rows of the tile
-    auto* op = as_global(out) + om.base(line0);
-    {
+    if constexpr ( TILE_OUT ) {
+        __syncthreads( );
+        auto*     op = as_global(out) + om.base(line0);
         const int l = threadIdx.x % LT, r = threadIdx.x / LT;
         for ( int j = r; j < M; j += 512 / LT )
             store_cpx(op, (long)j * om.es + l, s_tile[l * PITCH + j]);
