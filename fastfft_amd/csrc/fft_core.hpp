@@ -131,6 +131,17 @@ __device__ __forceinline__ global_t<T>* as_global(T* p) {
     return (global_t<T>*)p;
 }
 
+// Launch descriptors, read-only while a kernel runs and read at wave-uniform
+// addresses, are read through the constant address space: scalar loads into
+// scalar registers.
+template <typename T>
+using const_t = const __attribute__((address_space(4))) T;
+
+template <typename T>
+__device__ __forceinline__ const_t<T>* as_const(const T* p) {
+    return (const_t<T>*)p;
+}
+
 // (half precision values move as their bit patterns: the half types do not
 // copy from an address-space-qualified object)
 __device__ __forceinline__ float load_real(const global_t<float>* p, long i) { return p[i]; }

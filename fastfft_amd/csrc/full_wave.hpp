@@ -33,17 +33,29 @@ __global__ void __launch_bounds__(64 * Q)
     const int     t    = threadIdx.x % T;
     const int     line = xcd_row(blockIdx.x, nlines);
     const float2* twm  = multi_twiddles(tw);
-    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    // ---- the line into LDS (natural order)
-    const TIn* lp = in + im.base(line);
-    if constexpr ( REAL ) {
-        for ( int n = threadIdx.x; n < N; n += NT )
-            s_real[n] = load_real(lp, n * im.es) * pre_scale;
-    }
-    else {
-        for ( int n = threadIdx.x; n < N; n += NT ) {
-            const float2 z = load_cpx(lp, n * im.es);
-            lds[n]         = make_float2(z.x * pre_scale, z.y * pre_scale);
+    // ---- the line into LDS (natural order), every load of the thread in
+    // flight before its LDS writes
+    {
+        RowTwiddleStage<P, Q> tws;
+        tws.load(twm);
+        const auto* lp = as_global(in) + im.base(line);
+        float2      z[N / NT];
+#pragma unroll
+        for ( int r = 0; r < N / NT; r++ ) {
+            const int n = threadIdx.x + NT * r;
+            if constexpr ( REAL )
+                z[r] = make_float2(load_real(lp, n * im.es), 0.f);
+            else
+                z[r] = load_cpx(lp, n * im.es);
+        }
+        tws.store(s_twp, s_twn);
+#pragma unroll
+        for ( int r = 0; r < N / NT; r++ ) {
+            const int n = threadIdx.x + NT * r;
+            if constexpr ( REAL )
+                s_real[n] = z[r].x * pre_scale;
+            else
+                lds[n] = make_float2(z[r].x * pre_scale, z[r].y * pre_scale);
         }
     }
     __syncthreads( );
@@ -127,15 +139,31 @@ __global__ void __launch_bounds__(64 * Q)
     const int     t    = threadIdx.x % T;
     const int     line = xcd_row(blockIdx.x, nlines);
     const float2* twm  = multi_twiddles(tw);
-    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
     // ---- the input line into LDS, decimated by Q, two values per lane (the
-    // decimated layout is conflict-free for pairs)
-    const TIn* lp   = in + im.base(line);
-    constexpr int KN = HERM ? H + 1 : N;
-    for ( int m = threadIdx.x; 2 * m < KN; m += NT ) {
-        lds[dec_slot<P, Q>(2 * m)] = load_cpx(lp, 2 * m * im.es);
-        if ( 2 * m + 1 < KN )
-            lds[dec_slot<P, Q>(2 * m + 1)] = load_cpx(lp, (2 * m + 1) * im.es);
+    // decimated layout is conflict-free for pairs), every load of the thread
+    // in flight before its LDS writes
+    {
+        RowTwiddleStage<P, Q> tws;
+        tws.load(twm);
+        const auto*   lp = as_global(in) + im.base(line);
+        constexpr int KN = HERM ? H + 1 : N;
+        constexpr int MR = (KN + 2 * NT - 1) / (2 * NT);
+        float2        u[MR][2];
+#pragma unroll
+        for ( int r = 0; r < MR; r++ ) {
+            const int m = threadIdx.x + NT * r;
+            u[r][0]     = 2 * m < KN ? load_cpx(lp, (long)(2 * m) * im.es) : make_float2(0.f, 0.f);
+            u[r][1]     = 2 * m + 1 < KN ? load_cpx(lp, (long)(2 * m + 1) * im.es) : make_float2(0.f, 0.f);
+        }
+        tws.store(s_twp, s_twn);
+#pragma unroll
+        for ( int r = 0; r < MR; r++ ) {
+            const int m = threadIdx.x + NT * r;
+            if ( 2 * m < KN )
+                lds[dec_slot<P, Q>(2 * m)] = u[r][0];
+            if ( 2 * m + 1 < KN )
+                lds[dec_slot<P, Q>(2 * m + 1)] = u[r][1];
+        }
     }
     __syncthreads( );
     FFT fft;

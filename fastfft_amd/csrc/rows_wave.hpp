@@ -40,6 +40,29 @@ __device__ __forceinline__ int xcd_row(int b, int nrows) {
     return (b % kXcds) * (nrows / kXcds) + b / kXcds;
 }
 
+// Descriptor i of a batched launch (i wave-uniform), by scalar loads.
+__device__ __forceinline__ RowBatchItem load_item(const RowBatchItem* items, int i) {
+    const auto*  p = as_const(items + i);
+    RowBatchItem r;
+    r.in    = p->in;
+    r.out   = p->out;
+    r.keys  = p->keys;
+    r.in_ld = p->in_ld;
+    r.L     = p->L;
+    r.rows  = p->rows;
+    return r;
+}
+
+__device__ __forceinline__ XcorrItem load_item(const XcorrItem* items, int i) {
+    const auto* p = as_const(items + i);
+    XcorrItem   r;
+    r.a      = p->a;
+    r.out    = p->out;
+    r.L      = p->L;
+    r.unused = 0;
+    return r;
+}
+
 // LDS slot of element k = Q k' + q of a row: the Q decimated sequences one after
 // another, each XOR-swizzled inside blocks of 32 slots, so that wave q's reads
 // or writes of consecutive k' and the row-order accesses of two consecutive k
@@ -57,11 +80,27 @@ __device__ __forceinline__ int dec_slot(int k) {
 // every wave reads consecutive slots: s_twp = the Stockham pass table of
 // WaveFFT<P> (init_pt), s_twn[64 q + t] = W_N^{t q} (lane_twiddle).
 template <int P, int Q>
+struct RowTwiddleStage {
+    static constexpr int NT = 64 * Q, NR = WaveFFT<P>::template pass_table_rows<NT>;
+    float2               pt[NR];
+    float2               ln;
+    // all loads of a thread (a workgroup of NT threads), so that the caller's
+    // own loads can be issued before the stores
+    __device__ __forceinline__ void load(const float2* __restrict__ twm) {
+        WaveFFT<P>::template load_pass_table<NT>(pt, twm, threadIdx.x);
+        ln = load_cpx(as_global(twm), P * Q + (threadIdx.x & 63) * (threadIdx.x >> 6));
+    }
+    __device__ __forceinline__ void store(float2* s_twp, float2* s_twn) const {
+        WaveFFT<P>::template store_pass_table<NT>(s_twp, pt, threadIdx.x);
+        s_twn[threadIdx.x] = ln;
+    }
+};
+
+template <int P, int Q>
 __device__ __forceinline__ void stage_row_twiddles(float2* s_twp, float2* s_twn, const float2* __restrict__ twm) {
-    constexpr int N = P * Q, NT = 64 * Q;
-    WaveFFT<P>::fill_pass_table(s_twp, twm, threadIdx.x, NT);
-    for ( int k = threadIdx.x; k < NT; k += NT )
-        s_twn[k] = twm[N + (k & 63) * (k >> 6)];
+    RowTwiddleStage<P, Q> st;
+    st.load(twm);
+    st.store(s_twp, s_twn);
 }
 
 // W_N^{t q} of lane t of wave q from the table of stage_row_twiddles
@@ -85,7 +124,7 @@ __global__ void __launch_bounds__(64 * Q)
     const int     row = xcd_row(blockIdx.x, nrows);
     const float2* twm = multi_twiddles(tw);
     if ( items ) {
-        const RowBatchItem it = items[blockIdx.y];
+        const RowBatchItem it = load_item(items, blockIdx.y);
         if ( row >= it.rows )
             return;
         in    = static_cast<const TIn*>(it.in);
@@ -93,10 +132,26 @@ __global__ void __launch_bounds__(64 * Q)
         im.s1 = it.in_ld;
         L     = it.L;
     }
-    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    const auto* rp = as_global(in) + im.base(row);
-    for ( int n = threadIdx.x; n < L; n += NT )
-        s_row[n] = load_real(rp, n * im.es) * pre_scale;
+    RowTwiddleStage<P, Q> tws;
+    tws.load(twm);
+    {
+        // L <= P: every load of the thread in flight before the LDS writes
+        constexpr int RR = (P + NT - 1) / NT;
+        const auto*   rp = as_global(in) + im.base(row);
+        float         xr[RR];
+#pragma unroll
+        for ( int r = 0; r < RR; r++ ) {
+            const int n = threadIdx.x + NT * r;
+            xr[r]       = n < L ? load_real(rp, n * im.es) : 0.f;
+        }
+        tws.store(s_twp, s_twn);
+#pragma unroll
+        for ( int r = 0; r < RR; r++ ) {
+            const int n = threadIdx.x + NT * r;
+            if ( n < L )
+                s_row[n] = xr[r] * pre_scale;
+        }
+    }
     __syncthreads( );
     FFT fft;
     fft.init_pt(t, s_twp);
@@ -176,32 +231,48 @@ __global__ void __launch_bounds__(64 * Q)
     const int     row = xcd_row(blockIdx.x, nrows);
     const float2* twm = multi_twiddles(tw);
     if ( items ) {
-        const RowBatchItem it = items[blockIdx.y];
+        const RowBatchItem it = load_item(items, blockIdx.y);
         in                    = static_cast<const float2*>(it.in);
         out                   = static_cast<TOut*>(it.out);
         row_keys              = it.keys;
     }
-    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    const auto*   cp     = as_global(in) + im.base(row);
-    const bool    staged = im.es == 1;
-    if ( staged ) {
+    RowTwiddleStage<P, Q> tws;
+    tws.load(twm);
+    const auto* cp     = as_global(in) + im.base(row);
+    const bool  staged = im.es == 1;
+    {
         // two values per lane (the decimated layout is conflict-free for pairs),
-        // 16-byte loads when the row start is 16-byte aligned
-        const bool a16 = (reinterpret_cast<unsigned long>(cp) & 15) == 0;
-        for ( int m = threadIdx.x; 2 * m <= H; m += NT ) {
-            float2 u0, u1;
-            if ( a16 && 2 * m + 1 <= H ) {
-                const float4 f = load_cpx_pair(cp, m);
-                u0             = make_float2(f.x, f.y);
-                u1             = make_float2(f.z, f.w);
+        // 16-byte loads when the row start is 16-byte aligned; all of a thread's
+        // loads in flight before its LDS writes
+        constexpr int MR  = (H / 2 + NT) / NT; // pairs m = 0 .. H/2
+        const bool    a16 = (reinterpret_cast<unsigned long>(cp) & 15) == 0;
+        float2        u0[MR], u1[MR];
+        if ( staged ) {
+#pragma unroll
+            for ( int r = 0; r < MR; r++ ) {
+                const int m = threadIdx.x + NT * r;
+                u0[r] = u1[r] = make_float2(0.f, 0.f);
+                if ( a16 && 2 * m + 1 <= H ) {
+                    const float4 f = load_cpx_pair(cp, m);
+                    u0[r]          = make_float2(f.x, f.y);
+                    u1[r]          = make_float2(f.z, f.w);
+                }
+                else if ( 2 * m <= H ) {
+                    u0[r] = load_cpx(cp, 2 * m);
+                    u1[r] = 2 * m + 1 <= H ? load_cpx(cp, 2 * m + 1) : make_float2(0.f, 0.f);
+                }
             }
-            else {
-                u0 = load_cpx(cp, 2 * m);
-                u1 = 2 * m + 1 <= H ? load_cpx(cp, 2 * m + 1) : make_float2(0.f, 0.f);
+        }
+        tws.store(s_twp, s_twn);
+        if ( staged ) {
+#pragma unroll
+            for ( int r = 0; r < MR; r++ ) {
+                const int m = threadIdx.x + NT * r;
+                if ( 2 * m <= H )
+                    lds[dec_slot<P, Q>(2 * m)] = u0[r];
+                if ( 2 * m + 1 <= H )
+                    lds[dec_slot<P, Q>(2 * m + 1)] = u1[r];
             }
-            lds[dec_slot<P, Q>(2 * m)] = u0;
-            if ( 2 * m + 1 <= H )
-                lds[dec_slot<P, Q>(2 * m + 1)] = u1;
         }
     }
     __syncthreads( );

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(512)
     const int     t     = threadIdx.x % T;
     const long    line0 = (long)blockIdx.x * LT;
     const float2* twm   = multi_twiddles(tw);
-    FFT::fill_pass_table(s_twp, twm, threadIdx.x, 512);
+    FFT::template fill_pass_table<512>(s_twp, twm, threadIdx.x);
     // values stored per line: N/2 + 1 of a Hermitian line, the non-zero inputs of a forward, else N
     const int kin = HERM ? H + 1 : (DIR < 0 ? (L < N ? L : N) : N);
     if constexpr ( TILE_IN ) {

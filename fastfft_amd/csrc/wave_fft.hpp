@@ -199,18 +199,57 @@ struct WaveFFT {
     static constexpr int PT = pass_table_size( );
     static_assert(PT <= P, "the pass table fits a P-entry buffer");
 
-    // Fills the pass table pt[0 .. PT) from the multi-resolution twiddle table
-    // (twm[n + m] = W_n^m), threads `first`, `first + stride`, ... of a workgroup.
+    // Index into the multi-resolution twiddle table (twm[n + m] = W_n^m) of
+    // pass-table entry g < PT: entry k of the pass with span Ns and radix R is
+    // W_{Ns R}^{(k mod Ns)(1 + k / Ns)}.
     template <int Ns = 1, int OFF = 0>
-    __device__ __forceinline__ static void fill_pass_table(float2* pt, const float2* __restrict__ twm, int first, int stride) {
+    __device__ __forceinline__ static int pass_table_index(int g) {
         if constexpr ( Ns < P ) {
-            constexpr int R = (P / Ns) < E ? (P / Ns) : E;
-            if constexpr ( Ns > 1 ) {
-                for ( int k = first; k < Ns * (R - 1); k += stride )
-                    pt[OFF + k] = twm[Ns * R + (k & (Ns - 1)) * (1 + k / Ns)];
+            constexpr int R  = (P / Ns) < E ? (P / Ns) : E;
+            constexpr int SZ = Ns > 1 ? Ns * (R - 1) : 0;
+            if ( SZ > 0 && g < OFF + SZ ) {
+                const int k = g - OFF;
+                return Ns * R + (k & (Ns - 1)) * (1 + k / Ns);
             }
-            fill_pass_table<Ns * R, OFF + (Ns > 1 ? Ns * (R - 1) : 0)>(pt, twm, first, stride);
+            return pass_table_index<Ns * R, OFF + SZ>(g);
         }
+        else {
+            return 0;
+        }
+    }
+
+    // The pass table pt[0 .. PT) is filled by a workgroup of STRIDE threads,
+    // thread `first` taking entries first + STRIDE r: load_pass_table issues
+    // all of a thread's loads, store_pass_table writes them, so that callers
+    // can issue their other loads in between.
+    template <int STRIDE>
+    static constexpr int pass_table_rows = PT > 0 ? (PT + STRIDE - 1) / STRIDE : 1;
+
+    template <int STRIDE>
+    __device__ __forceinline__ static void load_pass_table(float2 (&v)[pass_table_rows<STRIDE>],
+                                                           const float2* __restrict__ twm, int first) {
+#pragma unroll
+        for ( int r = 0; r < pass_table_rows<STRIDE>; r++ ) {
+            const int g = first + STRIDE * r;
+            v[r]        = g < PT ? load_cpx(as_global(twm), pass_table_index(g)) : make_float2(0.f, 0.f);
+        }
+    }
+
+    template <int STRIDE>
+    __device__ __forceinline__ static void store_pass_table(float2* pt, const float2 (&v)[pass_table_rows<STRIDE>], int first) {
+#pragma unroll
+        for ( int r = 0; r < pass_table_rows<STRIDE>; r++ ) {
+            const int g = first + STRIDE * r;
+            if ( g < PT )
+                pt[g] = v[r];
+        }
+    }
+
+    template <int STRIDE>
+    __device__ __forceinline__ static void fill_pass_table(float2* pt, const float2* __restrict__ twm, int first) {
+        float2 v[pass_table_rows<STRIDE>];
+        load_pass_table<STRIDE>(v, twm, first);
+        store_pass_table<STRIDE>(pt, v, first);
     }
 
     template <int Ns, int PI, int OFF>

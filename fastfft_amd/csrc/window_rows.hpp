@@ -34,15 +34,32 @@ __global__ void __launch_bounds__(64 * Q)
     const int     t    = threadIdx.x % T;
     const int     line = xcd_row(blockIdx.x, nlines);
     const float2* twm  = multi_twiddles(tw);
-    stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    s_w1[threadIdx.x] = twm[N + threadIdx.x];
-    // ---- the input line into LDS, decimated by Q, two values per lane
-    const TIn*    lp = in + im.base(line);
-    constexpr int KN = HERM ? H + 1 : N;
-    for ( int m = threadIdx.x; 2 * m < KN; m += NT ) {
-        lds[dec_slot<P, Q>(2 * m)] = load_cpx(lp, (long)(2 * m) * im.es);
-        if ( 2 * m + 1 < KN )
-            lds[dec_slot<P, Q>(2 * m + 1)] = load_cpx(lp, (long)(2 * m + 1) * im.es);
+    // ---- the input line into LDS, decimated by Q, two values per lane, every
+    // load of the thread in flight before its LDS writes
+    {
+        RowTwiddleStage<P, Q> tws;
+        tws.load(twm);
+        const float2  w1 = load_cpx(as_global(twm), N + threadIdx.x);
+        const auto*   lp = as_global(in) + im.base(line);
+        constexpr int KN = HERM ? H + 1 : N;
+        constexpr int MR = (KN + 2 * NT - 1) / (2 * NT);
+        float2        u[MR][2];
+#pragma unroll
+        for ( int r = 0; r < MR; r++ ) {
+            const int m = threadIdx.x + NT * r;
+            u[r][0]     = 2 * m < KN ? load_cpx(lp, (long)(2 * m) * im.es) : make_float2(0.f, 0.f);
+            u[r][1]     = 2 * m + 1 < KN ? load_cpx(lp, (long)(2 * m + 1) * im.es) : make_float2(0.f, 0.f);
+        }
+        tws.store(s_twp, s_twn);
+        s_w1[threadIdx.x] = w1;
+#pragma unroll
+        for ( int r = 0; r < MR; r++ ) {
+            const int m = threadIdx.x + NT * r;
+            if ( 2 * m < KN )
+                lds[dec_slot<P, Q>(2 * m)] = u[r][0];
+            if ( 2 * m + 1 < KN )
+                lds[dec_slot<P, Q>(2 * m + 1)] = u[r][1];
+        }
     }
     __syncthreads( );
     FFT fft;

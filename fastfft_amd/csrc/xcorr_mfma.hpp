@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256)
     // the template column of item ii: a[32 n1 + n2] at lane n2, element n1 - 8 h
     // (issued one item ahead, so that the strided loads overlap the transforms)
     auto gather = [&](int ii, float2 (&z)[8]) {
-        const XcorrItem it = items ? items[ii] : one;
+        const XcorrItem it = items ? load_item(items, ii) : one;
         const auto*     a  = as_global(static_cast<const float2*>(it.a)) + col;
 #pragma unroll
         for ( int e = 0; e < 8; e++ ) {
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256)
     if ( first + w < last )
         gather(first + w, znext);
     for ( int ii = first + w; ii < last; ii += kMfmaWaves ) {
-        auto* out = as_global(static_cast<float2*>((items ? items[ii] : one).out)) + col;
+        auto* out = as_global(static_cast<float2*>((items ? load_item(items, ii) : one).out)) + col;
         // ---- forward stage 1: B[n1][n2] = a[32 n1 + n2], n1 = 8 h + e
         f32x16 cr = { }, ci = { };
         {
