@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 SOURCES = ["planner.cpp", "capi.cpp", "launch_fwd.hip", "launch_inv.hip", "launch_xcorr.hip", "launch_tile.hip", "misc.hip"]
 HEADERS = ["fft_core.hpp", "line_kernels.hpp", "launch.hpp", "launch_common.hpp", "plan.hpp", "wave_fft.hpp",
-           "xcorr_col.hpp", "rows_wave.hpp", "items.hpp", "full_wave.hpp", "window_rows.hpp", "mfma_dft.hpp", "xcorr_mfma.hpp", "tile_lines.hpp"]
+           "xcorr_col.hpp", "rows_wave.hpp", "items.hpp", "full_wave.hpp", "window_rows.hpp", "mfma_dft.hpp", "xcorr_mfma.hpp", "tile_lines.hpp", "rows_packed.hpp"]
 
 
 def _needs(obj: str, src: str) -> bool:

@@ -3,6 +3,7 @@
 
 #include "full_wave.hpp"
 #include "launch_common.hpp"
+#include "rows_packed.hpp"
 #include "rows_wave.hpp"
 
 namespace fastfft_amd {
@@ -37,6 +38,34 @@ static hipError_t go_r2c_wave_batch(const Stage& s, const RowBatchItem* items, i
     return hipGetLastError( );
 }
 
+// R2C rows of 512 or 1024 real points, one wave per row (rows_packed.hpp); the
+// batched form takes its per-template in, pitch, L and row count from items.
+static bool r2c_packed_covers(const Stage& s) {
+    return s.kind == StageKind::fwd_real && (s.N == 512 || s.N == 1024) && s.nout >= 1 && s.nout <= s.N / 2 + 1 &&
+           s.L >= 0 && s.L <= s.N && ! env_int("FASTFFT_NO_PACKED_ROWS", 0);
+}
+
+template <int P, typename TIn>
+static hipError_t go_r2c_packed(const Stage& s, const void* src, void* dst, int nrows, float pre, const float2* tw,
+                                const RowBatchItem* items, int n, hipStream_t st) {
+    const dim3 grid((unsigned)((nrows + kPackedRows - 1) / kPackedRows), (unsigned)n);
+    k_r2c_rows_packed<P, TIn><<<grid, dim3(64 * kPackedRows), 0, st>>>(static_cast<const TIn*>(src), to_device(s.im), s.L,
+                                                                      static_cast<float2*>(dst), to_device(s.om), s.nout,
+                                                                      nrows, pre, tw, items);
+    return hipGetLastError( );
+}
+
+static hipError_t launch_r2c_packed(const Stage& s, const void* src, void* dst, RealT rt, int nrows, float pre,
+                                    const float2* tw, const RowBatchItem* items, int n, hipStream_t st) {
+    if ( ! r2c_packed_covers(s) || nrows < 1 || n < 1 )
+        return hipErrorNotSupported;
+    if ( s.N == 512 )
+        return rt == RealT::f32 ? go_r2c_packed<256, float>(s, src, dst, nrows, pre, tw, items, n, st)
+                                : go_r2c_packed<256, __half>(s, src, dst, nrows, pre, tw, items, n, st);
+    return rt == RealT::f32 ? go_r2c_packed<512, float>(s, src, dst, nrows, pre, tw, items, n, st)
+                            : go_r2c_packed<512, __half>(s, src, dst, nrows, pre, tw, items, n, st);
+}
+
 // (P, Q) of an R2C row stage that k_r2c_rows_wave covers
 static bool r2c_rows_shape(const Stage& s) {
     return (s.P == 256 || s.P == 512 || s.P == 1024) && (s.Q == 1 || s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) &&
@@ -45,15 +74,21 @@ static bool r2c_rows_shape(const Stage& s) {
 
 bool r2c_rows_batch_covers(const Stage& s) {
     // one template per grid row: single-index lines, unit-stride row-major output with even leading dimension
-    return r2c_rows_shape(s) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1 && s.om.es == 1 && s.om.s1 % 2 == 0;
+    return (r2c_rows_shape(s) || r2c_packed_covers(s)) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1 && s.om.es == 1 &&
+           s.om.s1 % 2 == 0;
 }
 
 hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
                                  const float2* tw, hipStream_t st) {
     if ( ! r2c_rows_batch_covers(s) )
         return hipErrorNotSupported;
-    const auto* it   = static_cast<const RowBatchItem*>(items);
-    auto        by_p = [&](auto pc) -> hipError_t {
+    const auto* it = static_cast<const RowBatchItem*>(items);
+    {
+        hipError_t e = launch_r2c_packed(s, nullptr, nullptr, in_rt, max_rows, pre, tw, it, n, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
+    auto by_p = [&](auto pc) -> hipError_t {
         constexpr int P  = decltype(pc)::value;
         auto          go = [&](auto qc) -> hipError_t {
             constexpr int Q = decltype(qc)::value;
@@ -150,6 +185,11 @@ hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipSt
     // a unit-stride output row is written with 16-byte stores: even line offsets, aligned base
     const bool rows_ok = s.om.es != 1 || (s.om.s1 % 2 == 0 && s.om.s2 % 2 == 0 &&
                                           reinterpret_cast<std::uintptr_t>(io.dst) % 16 == 0);
+    if ( io.src_real && ! io.dst_real && io.dst_ct == CpxT::c64 && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = launch_r2c_packed(s, io.src, io.dst, io.src_rt, s.nlines, io.scale, tw, nullptr, 1, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     if ( rows_ok && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) && ! env_int("FASTFFT_NO_FULL_LINES", 0) ) {
         hipError_t e = try_full_wave(s, io, tw, st);
         if ( e != hipErrorNotSupported )

@@ -1,6 +1,7 @@
 // Inverse line stages: C2C y/z passes (output pruned) and the C2R x-pass.
 #include "full_wave.hpp"
 #include "launch_common.hpp"
+#include "rows_packed.hpp"
 #include "rows_wave.hpp"
 #include "window_rows.hpp"
 
@@ -17,6 +18,49 @@ static hipError_t go_inv(const Stage& s, const StageIO& io, const float2* tw, hi
                                                                   static_cast<TOut*>(io.dst), to_device(s.om), s.N, s.M,
                                                                   s.o, s.Q, s.nlines, io.scale, tw, twiddle_stride(s.N));
     return hipGetLastError( );
+}
+
+// C2R rows of 512 or 1024 real points, any window of M <= N outputs, one wave
+// per row (rows_packed.hpp); keys: one argmax key per row, or null.
+static bool c2r_packed_covers(const Stage& s) {
+    return s.kind == StageKind::inv_c2r && (s.N == 512 || s.N == 1024) && s.M >= 1 && s.M <= s.N &&
+           ! env_int("FASTFFT_NO_PACKED_ROWS", 0);
+}
+
+template <int P, typename TOut>
+static hipError_t go_c2r_packed(const Stage& s, const void* src, void* dst, float post, const float2* tw,
+                                unsigned long long* keys, const RowBatchItem* items, int n, hipStream_t st) {
+    const dim3 grid((unsigned)((s.nlines + kPackedRows - 1) / kPackedRows), (unsigned)n);
+    k_c2r_rows_packed<P, TOut><<<grid, dim3(64 * kPackedRows), 0, st>>>(static_cast<const float2*>(src), to_device(s.im),
+                                                                       static_cast<TOut*>(dst), to_device(s.om), s.M, s.o,
+                                                                       s.nlines, post, tw, keys, items);
+    return hipGetLastError( );
+}
+
+static hipError_t launch_c2r_packed(const Stage& s, const void* src, void* dst, RealT rt, float post, const float2* tw,
+                                    unsigned long long* keys, const RowBatchItem* items, int n, hipStream_t st) {
+    if ( ! c2r_packed_covers(s) || s.nlines < 1 || n < 1 )
+        return hipErrorNotSupported;
+    if ( s.N == 512 )
+        return rt == RealT::f32 ? go_c2r_packed<256, float>(s, src, dst, post, tw, keys, items, n, st)
+                                : go_c2r_packed<256, __half>(s, src, dst, post, tw, keys, items, n, st);
+    return rt == RealT::f32 ? go_c2r_packed<512, float>(s, src, dst, post, tw, keys, items, n, st)
+                            : go_c2r_packed<512, __half>(s, src, dst, post, tw, keys, items, n, st);
+}
+
+// single C2R pass on the packed kernel, with its argmax keys reduced after it
+static hipError_t try_c2r_packed(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( ! c2r_packed_covers(s) || io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real )
+        return hipErrorNotSupported;
+    const bool keys = io.amax && s.nlines <= kArgmaxPartials;
+    hipError_t e    = launch_c2r_packed(s, io.src, io.dst, io.dst_rt, io.scale, tw, keys ? io.amax->keys : nullptr,
+                                        nullptr, 1, st);
+    if ( e == hipSuccess && keys ) {
+        e = launch_argmax_keys(io.amax->keys, s.nlines, s.M, io.amax->value, io.amax->xy, st);
+        if ( e == hipSuccess && io.amax_fused )
+            *io.amax_fused = true;
+    }
+    return e;
 }
 
 // Trimmed C2R rows with P in {256, 512, 1024} and a window origin at register
@@ -93,16 +137,22 @@ static int c2r_rows_rotation(const Stage& s) {
 }
 
 bool c2r_rows_batch_covers(const Stage& s) {
-    return c2r_rows_rotation(s) >= 0 && s.nlines <= kArgmaxPartials && s.im.n2 == 1 && s.om.n2 == 1;
+    return (c2r_rows_rotation(s) >= 0 || c2r_packed_covers(s)) && s.nlines <= kArgmaxPartials && s.im.n2 == 1 &&
+           s.om.n2 == 1;
 }
 
 hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,
                                  hipStream_t st) {
     if ( ! c2r_rows_batch_covers(s) )
         return hipErrorNotSupported;
-    const auto* it   = static_cast<const RowBatchItem*>(items);
-    const int   rs   = c2r_rows_rotation(s);
-    auto        by_p = [&](auto pc) -> hipError_t {
+    const auto* it = static_cast<const RowBatchItem*>(items);
+    {
+        hipError_t e = launch_c2r_packed(s, nullptr, nullptr, out_rt, post, tw, nullptr, it, n, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
+    const int rs = c2r_rows_rotation(s);
+    auto      by_p = [&](auto pc) -> hipError_t {
         constexpr int P  = decltype(pc)::value;
         constexpr int E  = P / 64;
         auto          go = [&](auto qc) -> hipError_t {
@@ -216,6 +266,11 @@ static hipError_t try_window(const Stage& s, const StageIO& io, const float2* tw
 }
 
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+        hipError_t e = try_c2r_packed(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) && ! env_int("FASTFFT_NO_FULL_LINES", 0) && ! io.amax ) {
         hipError_t e = try_full_inv(s, io, tw, st);
         if ( e != hipErrorNotSupported )

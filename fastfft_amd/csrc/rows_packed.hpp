@@ -1,0 +1,192 @@
+// Row kernels for rows of N = 2P real values with one wavefront per row: the
+// even and odd samples of a row form one complex P-point sequence, so a row
+// costs one P-point transform, where the wave kernels of rows_wave.hpp spend
+// one per decimated half (two for N = 2P).  These are the x-passes of the
+// 2-D correlation, FwdFFT and InvFFT for 512- and 1024-point rows (the
+// reference's block_fft_kernel_R2C_INCREASE_XY / _NONE_XY and
+// C2R_DECREASE_XY / _NONE_XY, src/fastfft/FastFFT.cu:1055-1166, :1641-1692).
+//
+//   k_r2c_rows_packed  z[m] = x[2m] + i x[2m+1] (zero at and past L), Z = FFT_P(z),
+//                      X[k] = (Z[k] + conj Z[P-k]) / 2 - i W_N^k (Z[k] - conj Z[P-k]) / 2
+//                      for k = 0 .. P (Z[P] = Z[0]); the k < nout values are stored.
+//   k_c2r_rows_packed  Z[k] = (X[k] + conj X[P-k]) + i W_N^{-k} (X[k] - conj X[P-k]),
+//                      z = IFFT_P(Z), x[2m] = Re z[m], x[2m+1] = Im z[m] (the unnormalised
+//                      N-point inverse of the Hermitian X[0 .. P], with the imaginary
+//                      parts of X[0] and X[P] taken as zero, as the real part of the
+//                      inverse of the Hermitian extension does); output j < M is
+//                      x[(o + j) mod N].
+//
+// A workgroup holds kPackedRows rows, one per wave; the Stockham pass table is
+// staged once per workgroup.  After the only workgroup barrier every wave works
+// on its own row and its own LDS region.
+#pragma once
+
+#include "rows_wave.hpp"
+
+namespace fastfft_amd {
+
+constexpr int kPackedRows = 4;
+
+// Grid: ceil(nrows / kPackedRows) x templates.  items != nullptr: template
+// items[blockIdx.y] supplies in, out, the input pitch, L and its row count.
+template <int P, typename TIn>
+__global__ void __launch_bounds__(64 * kPackedRows)
+        k_r2c_rows_packed(const TIn* __restrict__ in, LineMap im, int L, float2* __restrict__ out, LineMap om, int nout,
+                          int nrows, float pre_scale, const float2* __restrict__ tw, const RowBatchItem* __restrict__ items) {
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows;
+    using FFT       = WaveFFT<P>;
+    __shared__ float2 lds[kPackedRows * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t   = threadIdx.x % T;
+    const float2* twm = multi_twiddles(tw);
+    if ( items ) {
+        const RowBatchItem it = load_item(items, blockIdx.y);
+        in                    = static_cast<const TIn*>(it.in);
+        out                   = static_cast<float2*>(it.out);
+        im.s1                 = it.in_ld;
+        L                     = it.L;
+        nrows                 = it.rows;
+    }
+    if ( (int)blockIdx.x * kPackedRows >= nrows ) // the whole workgroup, before any barrier
+        return;
+    const int  row  = (int)blockIdx.x * kPackedRows + w;
+    const bool live = row < nrows;
+    // ---- every load of the thread issued before its LDS writes
+    float2 ptv[FFT::template pass_table_rows<NT>];
+    FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+    const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
+    float2       v[E];
+    if ( live ) {
+        const auto* rp = as_global(in) + im.base(row);
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int n = 2 * (t + T * i);
+            v[i]        = make_float2(n < L ? load_real(rp, (long)n * im.es) : 0.f,
+                                      n + 1 < L ? load_real(rp, (long)(n + 1) * im.es) : 0.f);
+        }
+    }
+    FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+    __syncthreads( );
+    if ( ! live )
+        return;
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        v[i] = make_float2(v[i].x * pre_scale, v[i].y * pre_scale);
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    float2* wl = lds + w * FFT::SLOTS;
+    fft.template run<-1>(v, wl, t); // v[i] = Z[t + 64 i]
+    // Z[P - k] from the wave's region (its last exchange was read before these writes)
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        wl[FFT::Layout::slot(t + T * i)] = v[i];
+    wave_lds_fence( );
+    auto* op = as_global(out) + om.base(row);
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int    k  = t + T * i;
+        const float2 zp = wl[FFT::Layout::slot((P - k) & (P - 1))];
+        const float2 a  = make_float2(0.5f * (v[i].x + zp.x), 0.5f * (v[i].y - zp.y)); // (Z + conj Zp) / 2
+        const float2 b  = make_float2(0.5f * (v[i].x - zp.x), 0.5f * (v[i].y + zp.y)); // (Z - conj Zp) / 2
+        const float2 c  = cmul(cmul(twm[N + T * i], wt), b);                           // W_N^k b
+        if ( k < nout )
+            store_cpx(op, (long)k * om.es, make_float2(a.x + c.y, a.y - c.x)); // a - i c
+    }
+    // X[P] = Re Z[0] - Im Z[0]
+    if ( t == 0 && P < nout )
+        store_cpx(op, (long)P * om.es, make_float2(v[0].x - v[0].y, 0.f));
+}
+
+// Grid: ceil(nrows / kPackedRows) x templates.  row_keys == nullptr: no argmax;
+// else the largest argmax key (fft_core.hpp) of row r, index r * M + j, goes to
+// row_keys[r].  items != nullptr: template items[blockIdx.y] supplies in, out
+// and row_keys.
+template <int P, typename TOut>
+__global__ void __launch_bounds__(64 * kPackedRows)
+        k_c2r_rows_packed(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o,
+                          int nrows, float post_scale, const float2* __restrict__ tw,
+                          unsigned long long* __restrict__ row_keys, const RowBatchItem* __restrict__ items) {
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows;
+    using FFT       = WaveFFT<P>;
+    __shared__ float2 lds[kPackedRows * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t   = threadIdx.x % T;
+    const float2* twm = multi_twiddles(tw);
+    if ( items ) {
+        const RowBatchItem it = load_item(items, blockIdx.y);
+        in                    = static_cast<const float2*>(it.in);
+        out                   = static_cast<TOut*>(it.out);
+        row_keys              = it.keys;
+    }
+    if ( (int)blockIdx.x * kPackedRows >= nrows )
+        return;
+    const int  row  = (int)blockIdx.x * kPackedRows + w;
+    const bool live = row < nrows;
+    float2     ptv[FFT::template pass_table_rows<NT>];
+    FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+    const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
+    // X[k] and X[P - k], k = t + 64 i (the row holds X[0 .. P])
+    float2 x[E], xp[E];
+    if ( live ) {
+        const auto* ip = as_global(in) + im.base(row);
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int k = t + T * i;
+            x[i]        = load_cpx(ip, (long)k * im.es);
+            xp[i]       = load_cpx(ip, (long)(P - k) * im.es);
+        }
+    }
+    FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+    __syncthreads( );
+    if ( ! live )
+        return;
+    // the real part of the Hermitian extension's inverse ignores Im X[0] and Im X[P]
+    if ( t == 0 ) {
+        x[0].y  = 0.f;
+        xp[0].y = 0.f;
+    }
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const float2 e = make_float2(x[i].x + xp[i].x, x[i].y - xp[i].y); // X + conj Xp
+        const float2 d = make_float2(x[i].x - xp[i].x, x[i].y + xp[i].y); // X - conj Xp
+        const float2 c = cmulc(d, cmul(twm[N + T * i], wt));              // W_N^{-k} d
+        v[i]           = make_float2(e.x - c.y, e.y + c.x);               // e + i c
+    }
+    fft.template run<+1>(v, lds + w * FFT::SLOTS, t); // v[i] = z[t + 64 i]
+    auto*              op   = as_global(out) + om.base(row);
+    const int          oN   = ((o % N) + N) % N;
+    unsigned long long best = 0;
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int   n = 2 * (t + T * i);
+        const float y[2] = {v[i].x * post_scale, v[i].y * post_scale};
+#pragma unroll
+        for ( int h = 0; h < 2; h++ ) {
+            const int j = (n + h - oN + N) & (N - 1);
+            if ( j < M ) {
+                store_real(op, (long)j * om.es, y[h]);
+                if ( row_keys ) {
+                    // the value as stored (rounded when the output is half precision)
+                    const auto k = argmax_key(convert_real<TOut>(y[h]), (unsigned)(row * M + j));
+                    best         = k > best ? k : best;
+                }
+            }
+        }
+    }
+    if ( row_keys ) {
+#pragma unroll
+        for ( int off = 32; off > 0; off >>= 1 ) {
+            const unsigned long long other = __shfl_xor(best, off);
+            best                           = other > best ? other : best;
+        }
+        if ( t == 0 )
+            as_global(row_keys)[row] = best;
+    }
+}
+
+} // namespace fastfft_amd

@@ -212,11 +212,15 @@ FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (25
              ((256, 256), (64, 32))]
 
 
-@pytest.mark.parametrize("mode", ["default", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_packed", "generic"])
 @pytest.mark.parametrize("fin,fout", FWD_CASES)
 def test_fwd_random(fin, fout, mode, monkeypatch):
+    """Forward transforms against the FP64 oracle with the default kernels (512- and 1024-point real
+    rows on the packed one-wave-per-row kernel), without the packed rows, and generic only."""
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    if mode == "no_packed":
+        monkeypatch.setenv("FASTFFT_NO_PACKED_ROWS", "1")
     H.log(f"fwd random {fin} -> {fout} mode={mode}")
     img = rand_img(fin[0], fin[1], seed=fin[0] + 7 * fin[1])
     ft = H.make_plan((*fin, 1), (*fout, 1), (*fout, 1), (*fout, 1))
@@ -235,13 +239,16 @@ INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256,
              ((512, 256), (300, 256), (-7, 0))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_window", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_window", "no_packed", "generic"])
 @pytest.mark.parametrize("nin,nout,origin", INV_CASES)
 def test_inv_random(nin, nout, origin, mode, monkeypatch):
     """Trimmed inverses against the FP64 oracle with the default kernel choice, without the windowed
-    inverse-row kernel (the generic line kernel takes its shapes), and with the generic kernel only."""
+    inverse-row kernel (the generic line kernel takes its shapes), without the packed C2R rows, and
+    with the generic kernel only."""
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
+    if mode == "no_packed":
+        monkeypatch.setenv("FASTFFT_NO_PACKED_ROWS", "1")
     if mode == "no_window":
         monkeypatch.setenv("FASTFFT_NO_WINDOW_ROWS", "1")
     H.log(f"inv random {nin} -> {nout} at {origin} mode={mode}")
@@ -386,7 +393,9 @@ def test_xcorr_4096_384_trim_512_centred():
 
 # ------------------------------------------------------------------ fused argmax of FwdImageInvFFTArgMax
 ARGMAX_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((40, 200), (64, 2048), (24, 256), (-12, -128)),
-                ((100, 60), (256, 128), (100, 60), (-50, -30)), ((24, 1000), (32, 4096), (16, 1024), (0, -512))]
+                ((100, 60), (256, 128), (100, 60), (-50, -30)), ((24, 1000), (32, 4096), (16, 1024), (0, -512)),
+                # 1024- and 512-point rows: the packed C2R kernel with its per-row keys, odd origins
+                ((300, 40), (1024, 64), (512, 32), (-256, 0)), ((200, 30), (512, 64), (101, 20), (-37, 5))]
 
 
 def _argmax_case(tin, n, nout, origin, input_type="f32", seed=0):

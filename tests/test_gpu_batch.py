@@ -54,13 +54,16 @@ BATCHES = [([(128, 128), (256, 256), (384, 384), (128, 128), (384, 384)], (4096,
 FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "512x512"}
 
 
-@pytest.mark.parametrize("mode", ["tuned", "chunk2", "full_frame", "per_template"])
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "full_frame", "per_template"])
 @pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
 def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
-    """Batched kernels (chunk size from the tuner, or 2 to force several chunks) and the per-template
-    launches all agree with single-template correlations and give the argmax of their own outputs."""
+    """Batched kernels (chunk size from the tuner, or 2 to force several chunks, or the two-wave row
+    kernels in place of the packed ones) and the per-template launches all agree with single-template
+    correlations and give the argmax of their own outputs."""
     if mode == "chunk2":
         monkeypatch.setenv("FASTFFT_BATCH_CHUNK", "2")
+    if mode == "no_packed":
+        monkeypatch.setenv("FASTFFT_NO_PACKED_ROWS", "1")
     if mode == "per_template":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "full_frame":
@@ -68,7 +71,7 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
     H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
     form = ft.Describe("batch").splitlines()[0]
-    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2") else None
+    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed") else None
     assert f"frame={frame or 'full'}" in form, form
     ft.FwdImageInvFFTBatch(img)
     pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
