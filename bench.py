@@ -19,6 +19,9 @@ Beside the headline line the JSON holds:
               (fastfft_set_precision), its speedup over FP32 and its parity;
   uneven_batch 256 templates of mixed sizes (96 x 128^2, 96 x 256^2,
               64 x 384^2) in one call per step;
+  stats_2dtm  the headline batch in template-matching statistics form
+              (per-pixel maximum, its template, sum and sum of squares
+              updated in place of the maps), against the batch's own maps;
   config4_3d  the 3-D size changes: padded forward 64^3 -> 256^3 and trimmed
               inverse 512^3 -> 128^3 (centred), with their HBM fractions;
   single      one correlation at a time (fastfft_fwd_image_inv_argmax, the
@@ -177,6 +180,37 @@ def main() -> int:
             if m and int(m.group(1)) == chunk:
                 traffic = int(m.group(2))
     bft.SetStream(torch.cuda.current_stream())
+
+    # ---------------------------------------------------------------- template-matching statistics form:
+    # the same batch updating per-pixel maximum / its template / sum / sum of squares in place of the maps
+    bft.FwdImageInvFFTBatch(spec, intra_op=conj)  # the maps of every template, for the comparison
+    torch.cuda.synchronize()
+    pitch = 2 * (M // 2 + 1)
+    maps = torch.stack([o[: pitch * M].view(M, pitch)[:, :M] for o in outs])
+    want_mip, want_best = maps.max(dim=0)
+    st = [torch.zeros(M * M, dtype=dt, device=dev) for dt in (torch.float32, torch.int32, torch.float32, torch.float32)]
+    bft.SetBatchStats(*st, first_id=0)
+    bft.SetStream(cap)
+    g_st = capture(lambda: (bft.ResetBatchStats(), bft.FwdImageInvFFTBatch(spec, intra_op=conj)), 1)
+    st_reps = max(2, K // 4)
+    st_ms = time_graph(g_st, st_reps, warm=3) / st_reps
+    bft.SetStream(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    scale = float(maps.abs().max())
+    top2 = torch.topk(maps, 2, dim=0).values
+    clear = (top2[0] - top2[1]) > 1e-5 * scale
+    stats_2dtm = {
+        "value": round(B * 1000.0 / st_ms, 1),
+        "unit": "template correlations/s",
+        "ms_per_call": round(st_ms, 5),
+        "form": bft.Describe("batch").splitlines()[0],
+        "mip_max_err_rel": float(f"{float((st[0].view(M, M) - want_mip).abs().max()) / scale:.3e}"),
+        "best_agree_frac_clear_pixels": round(float((st[1].view(M, M) == want_best)[clear].float().mean()), 6),
+        "sum_max_err_rel": float(f"{float((st[2].view(M, M) - maps.sum(0)).abs().max()) / float(maps.abs().sum(0).max()):.3e}"),
+        "timing": f"HIP graph of ResetBatchStats + one FwdImageInvFFTBatch call in statistics form replayed {st_reps} times",
+    }
+    bft.SetBatchStats()
+    del maps, want_mip, want_best, st, top2, clear
 
     # ---------------------------------------------------------------- split-BF16 form
     sft, souts, spk, s_ms, _ = batch_form("split_bf16")
@@ -344,6 +378,7 @@ def main() -> int:
             "peaks_identical_to_fp32": s_peaks_same,
         },
         "uneven_batch": uneven,
+        "stats_2dtm": stats_2dtm,
         "config4_3d": config4,
         "single": {
             "value": round(1000.0 / single_ms, 1),

@@ -100,6 +100,8 @@ def lib():
         "fastfft_fwd_image_inv_argmax": [P, V, V, V, _Op, _Op, _Op, V, V],
         "fastfft_set_batch": [P, ctypes.c_int, V, V, V, V, V],
         "fastfft_fwd_image_inv_batch": [P, V, _Op, _Op, _Op],
+        "fastfft_set_batch_stats": [P, V, V, V, V, I],
+        "fastfft_reset_batch_stats": [P],
         "fastfft_wait": [P],
         "fastfft_copy_device_to_host_and_synchronize": [P, V, I],
         "fastfft_get_memory_sizes": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
@@ -277,6 +279,19 @@ class FourierTransformer:
         self._bind_torch_stream()
         _check(lib().fastfft_fwd_image_inv_batch(self._h, _ptr(image_to_search), pre_op or Noop(), intra_op or ConjMul(),
                                                  post_op or Noop()))
+
+    def SetBatchStats(self, mip=None, best=None, sum=None, sum_sq=None, first_id: int = 0) -> None:
+        """Extension (template matching, see fastfft_set_batch_stats): device maps of out_x * out_y
+        elements (mip, sum, sum_sq float32, best int32) that FwdImageInvFFTBatch updates in place of the
+        outputs: running per-pixel maximum, the template id (first_id + index) of its first occurrence,
+        sum and sum of squares.  All None switches the statistics off."""
+        self._stats_keepalive = (mip, best, sum, sum_sq)
+        _check(lib().fastfft_set_batch_stats(self._h, _ptr(mip), _ptr(best), _ptr(sum), _ptr(sum_sq), int(first_id)))
+
+    def ResetBatchStats(self) -> None:
+        """Extension: mip = -inf, best = -1, sum = sum_sq = 0, asynchronously on the plan's stream."""
+        self._bind_torch_stream()
+        _check(lib().fastfft_reset_batch_stats(self._h))
 
     def RunBatchStage(self, stage: int, image_to_search, pre_op=None, intra_op=None, post_op=None) -> None:
         """One part of the batched call over the whole batch, for timing (see fastfft_run_batch_stage):

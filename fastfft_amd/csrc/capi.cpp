@@ -93,6 +93,11 @@ struct fastfft_plan_s {
     std::vector<BatchItem>                       batch;
     int*                                         batch_peaks  = nullptr; // device int[2n] or null
     float*                                       batch_values = nullptr; // device float[n] or null
+    // template-matching statistics (fastfft_set_batch_stats): when on, the C2R part of
+    // a batched call updates these maps in place of the outputs and peaks
+    BatchStats batch_stats;
+    int        stats_first_id = 0;
+    bool       stats_on       = false;
     std::map<std::pair<int, int>, std::vector<Stage>> batch_stages;
     // batched kernels (when they cover the plan's stages): chunks of batch_chunk
     // templates share one launch per stage; batch_dev holds the A and B slots of
@@ -133,7 +138,16 @@ static int op_index(const char* op) {
 
 static const char* kOpNames[3] = {"fwd", "inv", "xcorr"};
 
+// the template-group scratch of the batch statistics, and the statistics setting
+static void clear_stats(fastfft_plan p) {
+    if ( p->batch_stats.part )
+        (void)hipFree(p->batch_stats.part);
+    p->batch_stats = BatchStats{ };
+    p->stats_on    = false;
+}
+
 static void free_batch_device(fastfft_plan p) {
+    clear_stats(p); // the statistics belong to the batch's kernels
     if ( p->batch_dev )
         (void)hipFree(p->batch_dev);
     p->batch_dev   = nullptr;
@@ -454,8 +468,10 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
             e          = launch_xcorr(st[1], io, tw, p->stream);
         }
         if ( e == hipSuccess && (parts & kBatchC2R) )
-            e = launch_c2r_rows_batch(st[2], p->batch_c2r + c0, cnt, rt, post, tw, p->stream);
-        if ( e == hipSuccess && p->batch_peaks && (parts & kBatchArgmax) )
+            e = p->stats_on ? launch_c2r_stats_batch(st[2], p->batch_c2r + c0, cnt, post, tw, p->stats_first_id + c0,
+                                                     p->batch_stats, p->stream)
+                            : launch_c2r_rows_batch(st[2], p->batch_c2r + c0, cnt, rt, post, tw, p->stream);
+        if ( e == hipSuccess && p->batch_peaks && ! p->stats_on && (parts & kBatchArgmax) )
             e = launch_argmax_keys(keys, st[2].nlines, st[2].M, p->batch_values + c0, p->batch_peaks + 2 * c0, p->stream, cnt,
                                    kArgmaxPartials);
         if ( e != hipSuccess )
@@ -943,6 +959,7 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
     plan->batch_stages = std::move(lists);
     plan->batch_peaks  = peaks_xy;
     plan->batch_values = values;
+    clear_stats(plan);
     // one launch per stage for chunks of templates when the fast kernels cover the plan's stages
     const std::vector<Stage>* sp = nullptr;
     if ( auto rc = get_stages(plan, 2, &sp) )
@@ -977,6 +994,8 @@ fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_
         return rc;
     if ( auto rc = ensure_validated(plan) )
         return rc;
+    if ( plan->stats_on && (plan->batch_chunk <= 0 || reinterpret_cast<std::uintptr_t>(image_to_search) % 16 != 0) )
+        return fail(FASTFFT_UNSUPPORTED, "batch statistics need the batched kernels and a 16-byte aligned image");
     if ( plan->batch_chunk > 0 && reinterpret_cast<std::uintptr_t>(image_to_search) % 16 == 0 ) {
         const std::vector<Stage>* sp = nullptr;
         if ( auto rc = get_stages(plan, 2, &sp) )
@@ -1192,6 +1211,65 @@ fastfft_status fastfft_run_stages(fastfft_plan plan, const char* op, int first, 
     return run_stages(plan, opi, in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op), op_scale(post_op), first, last);
 }
 
+// the C2R stage of the batch's active form (frame or full)
+static fastfft_status batch_c2r_stage(fastfft_plan p, const Stage** out) {
+    if ( p->batch.empty( ) || p->batch_chunk <= 0 )
+        return fail(FASTFFT_UNSUPPORTED, "batch statistics need a batch set on the batched kernels (fastfft_set_batch)");
+    if ( p->sub_active ) {
+        *out = &p->sub_xc[2];
+        return ok( );
+    }
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(p, 2, &sp) )
+        return rc;
+    if ( sp->size( ) != 3 )
+        return fail(FASTFFT_UNSUPPORTED, "batch statistics: unexpected stage list");
+    *out = &(*sp)[2];
+    return ok( );
+}
+
+fastfft_status fastfft_set_batch_stats(fastfft_plan plan, float* mip, int* best, float* sum, float* sum_sq, int first_id) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! mip && ! best && ! sum && ! sum_sq ) {
+        clear_stats(plan);
+        return ok( );
+    }
+    if ( ! mip || ! best || ! sum || ! sum_sq )
+        return fail(FASTFFT_INVALID_ARGUMENT, "set_batch_stats: mip, best, sum and sum_sq are all set or all null");
+    const Stage* c2r = nullptr;
+    if ( auto rc = batch_c2r_stage(plan, &c2r) )
+        return rc;
+    if ( ! c2r_stats_covers(*c2r) )
+        return fail(FASTFFT_UNSUPPORTED, "set_batch_stats: the statistics form covers 512- and 1024-point frames along x "
+                                         "(this batch runs " + std::to_string(c2r->N) + "-point rows)");
+    // partial maps of the template groups, allocated once per batch (calls allocate nothing)
+    float* part = plan->batch_stats.part;
+    if ( ! part ) {
+        const size_t bytes = sizeof(float) * 4 * (size_t)kStatsGroups * (size_t)c2r->M * (size_t)c2r->nlines;
+        if ( hipMalloc(&part, bytes) != hipSuccess ) {
+            (void)hipGetLastError( );
+            part = nullptr; // one group per launch
+        }
+    }
+    plan->batch_stats    = BatchStats{mip, best, sum, sum_sq, part};
+    plan->stats_first_id = first_id;
+    plan->stats_on       = true;
+    return ok( );
+}
+
+fastfft_status fastfft_reset_batch_stats(fastfft_plan plan) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! plan->stats_on )
+        return fail(FASTFFT_INVALID_ARGUMENT, "reset_batch_stats: no statistics set (fastfft_set_batch_stats)");
+    const Stage* c2r = nullptr;
+    if ( auto rc = batch_c2r_stage(plan, &c2r) )
+        return rc;
+    const hipError_t e = launch_stats_reset(plan->batch_stats, (long)c2r->M * c2r->nlines, plan->stream);
+    return e == hipSuccess ? ok( ) : hip_fail(e, "reset_batch_stats");
+}
+
 fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, size_t buf_len) {
     if ( auto rc = check_plan(plan) )
         return rc;
@@ -1212,6 +1290,8 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
         else {
             s += " frame=full\n";
         }
+        if ( plan->stats_on )
+            s.insert(s.find('\n'), " stats=on first_id=" + std::to_string(plan->stats_first_id));
     }
     else {
         const int opi = op_index(op);

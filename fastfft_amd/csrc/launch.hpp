@@ -63,6 +63,25 @@ hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int m
 hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,
                                  hipStream_t stream);
 
+// Per-pixel template-matching statistics of the batch's trimmed outputs (dense
+// M x nlines maps of the C2R stage, index row * M + j): running maximum, the
+// template id of its first occurrence, sum and sum of squares.
+struct BatchStats {
+    float* mip    = nullptr;
+    int*   best   = nullptr;
+    float* sum    = nullptr;
+    float* sum_sq = nullptr;
+    // kStatsGroups x 4 partial maps for template groups (null: one group per launch)
+    float* part = nullptr;
+};
+constexpr int kStatsGroups = 8;
+// the packed C2R rows (N = 512 or 1024) run the statistics form
+bool       c2r_stats_covers(const Stage& s);
+hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, float post, const float2* tw, int first_id,
+                                  const BatchStats& bs, hipStream_t stream);
+// mip = -inf, best = -1, sum = sum_sq = 0 over n positions
+hipError_t launch_stats_reset(const BatchStats& bs, long n, hipStream_t stream);
+
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);

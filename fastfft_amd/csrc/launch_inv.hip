@@ -48,6 +48,85 @@ static hipError_t launch_c2r_packed(const Stage& s, const void* src, void* dst, 
                             : go_c2r_packed<512, __half>(s, src, dst, post, tw, keys, items, n, st);
 }
 
+// Template-matching statistics of a chunk (k_c2r_stats_packed) and their reset
+// (a fill kernel, so that a captured graph holds no memset).
+bool c2r_stats_covers(const Stage& s) { return c2r_packed_covers(s) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1; }
+
+// Partial maps of template group g are merged in group order, after the maps
+// of earlier chunks and calls, so the first maximum keeps its template.
+__global__ void __launch_bounds__(256) k_stats_merge(const float* __restrict__ part, int G, long npix, float* __restrict__ mip,
+                                                     int* __restrict__ best, float* __restrict__ sum, float* __restrict__ sum_sq) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if ( i >= npix )
+        return;
+    const auto* pg = as_global(part);
+    float       m = as_global(mip)[i], a = as_global(sum)[i], q = as_global(sum_sq)[i];
+    int         b = as_global(best)[i];
+    for ( int g = 0; g < G; g++ ) {
+        const long  base = 4L * g * npix; // mip, best (as bits), sum, sum_sq of group g
+        const float v    = pg[base + i];
+        if ( v > m ) {
+            m = v;
+            b = __float_as_int(pg[base + npix + i]);
+        }
+        a += pg[base + 2 * npix + i];
+        q += pg[base + 3 * npix + i];
+    }
+    as_global(mip)[i]    = m;
+    as_global(best)[i]   = b;
+    as_global(sum)[i]    = a;
+    as_global(sum_sq)[i] = q;
+}
+
+hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, float post, const float2* tw, int first_id,
+                                  const BatchStats& bs, hipStream_t st) {
+    if ( ! c2r_stats_covers(s) || n < 1 )
+        return hipErrorNotSupported;
+    // template groups spread a chunk over the GPU when the scratch is there: 4 groups of at
+    // least 4 templates (fewer, larger groups write fewer partial maps), or groups of
+    // FASTFFT_STATS_PER_GROUP templates, at most kStatsGroups
+    const int tpg = env_int("FASTFFT_STATS_PER_GROUP", 0);
+    int       G   = tpg > 0 ? (n + tpg - 1) / tpg : std::min(4, (n + 3) / 4);
+    G             = bs.part ? std::max(1, std::min(kStatsGroups, G)) : 1;
+    const int per = (n + G - 1) / G;
+    G             = (n + per - 1) / per; // no empty group
+    const dim3  grid((unsigned)((s.nlines + kPackedRows - 1) / kPackedRows), (unsigned)G), block(64 * kPackedRows);
+    const auto* it = static_cast<const RowBatchItem*>(items);
+    auto        go = [&](auto pc, auto hc) {
+        k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value><<<grid, block, 0, st>>>(
+                to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq, bs.part);
+        return hipGetLastError( );
+    };
+    const bool half = 2 * s.M <= s.N;
+    using P256      = std::integral_constant<int, 256>;
+    using P512      = std::integral_constant<int, 512>;
+    hipError_t e    = s.N == 512 ? (half ? go(P256{ }, std::true_type{ }) : go(P256{ }, std::false_type{ }))
+                                 : (half ? go(P512{ }, std::true_type{ }) : go(P512{ }, std::false_type{ }));
+    if ( e != hipSuccess || G == 1 )
+        return e;
+    const long npix = (long)s.nlines * s.M;
+    k_stats_merge<<<dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st>>>(bs.part, G, npix, bs.mip, bs.best, bs.sum,
+                                                                             bs.sum_sq);
+    return hipGetLastError( );
+}
+
+__global__ void __launch_bounds__(256) k_stats_reset(float* mip, int* best, float* sum, float* sum_sq, long n) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if ( i < n ) {
+        as_global(mip)[i]    = -__builtin_huge_valf( );
+        as_global(best)[i]   = -1;
+        as_global(sum)[i]    = 0.f;
+        as_global(sum_sq)[i] = 0.f;
+    }
+}
+
+hipError_t launch_stats_reset(const BatchStats& bs, long n, hipStream_t st) {
+    if ( n < 1 )
+        return hipSuccess;
+    k_stats_reset<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(bs.mip, bs.best, bs.sum, bs.sum_sq, n);
+    return hipGetLastError( );
+}
+
 // single C2R pass on the packed kernel, with its argmax keys reduced after it
 static hipError_t try_c2r_packed(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( ! c2r_packed_covers(s) || io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real )

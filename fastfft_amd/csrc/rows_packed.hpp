@@ -189,4 +189,149 @@ __global__ void __launch_bounds__(64 * kPackedRows)
     }
 }
 
+// Template-matching statistics over a chunk of templates (no counterpart in the
+// reference; the per-pixel consumer of a 2DTM search): the wave of row r runs
+// the C2R of that row for items[0 .. nitems) in order and keeps, for each
+// window position p = r * M + j, the largest value and the first item reaching
+// it, the sum and the sum of squares, then updates
+//   mip[p] = max(mip[p], local max)  (best[p] = first_id + item when it rises),
+//   sum[p] += local sum, sum_sq[p] += local sum of squares.
+// Each transformed row goes through the wave's LDS region, so that lane t keeps
+// only window outputs j = 2 (t + 64 r) + h (HALF: M <= N/2, half the registers).
+// Every position belongs to one wave, so the update needs no atomics; chunks
+// launched in order on one stream see each other's results.  Input rows are
+// unit-stride (im.es == 1).  With gridDim.y = G > 1 template groups, group g
+// takes items [g per, (g + 1) per) and writes its statistics to its own
+// partial maps part + 4 g npix (mip, best, sum, sum_sq as 32-bit words) for
+// k_stats_merge.
+template <int P, bool HALF>
+__global__ void __launch_bounds__(64 * kPackedRows)
+        k_c2r_stats_packed(LineMap im, int M, int o, int nrows, float post_scale, const float2* __restrict__ tw,
+                           const RowBatchItem* __restrict__ items, int nitems, int first_id, float* __restrict__ mip,
+                           int* __restrict__ best, float* __restrict__ sum, float* __restrict__ sum_sq,
+                           float* __restrict__ part) {
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows, R = HALF ? E / 2 : E;
+    if ( gridDim.y > 1 ) {
+        const int  per  = (nitems + (int)gridDim.y - 1) / (int)gridDim.y;
+        const int  g0   = (int)blockIdx.y * per;
+        const long npix = (long)nrows * M;
+        items += g0;
+        first_id += g0;
+        nitems = min(per, nitems - g0);
+        mip    = part + 4 * blockIdx.y * npix;
+        best   = reinterpret_cast<int*>(mip + npix);
+        sum    = mip + 2 * npix;
+        sum_sq = mip + 3 * npix;
+    }
+    using FFT       = WaveFFT<P>;
+    __shared__ float2 lds[kPackedRows * FFT::SLOTS];
+    __shared__ float2 s_twp[P];
+    const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t   = threadIdx.x % T;
+    const float2* twm = multi_twiddles(tw);
+    if ( (int)blockIdx.x * kPackedRows >= nrows )
+        return;
+    const int  row  = (int)blockIdx.x * kPackedRows + w;
+    const bool live = row < nrows;
+    FFT::template fill_pass_table<NT>(s_twp, twm, threadIdx.x);
+    __syncthreads( );
+    if ( ! live )
+        return;
+    const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
+    FFT          fft;
+    fft.init_pt(t, s_twp);
+    float2*   wl = lds + w * FFT::SLOTS;
+    const int oN = ((o % N) + N) % N;
+    // per window position j = 2 (t + 64 r) + h: local max, its item, sum, sum of squares
+    float lmax[R][2], lsum[R][2], lsq[R][2];
+    int   litem[R][2];
+#pragma unroll
+    for ( int r = 0; r < R; r++ )
+#pragma unroll
+        for ( int h = 0; h < 2; h++ ) {
+            lmax[r][h]  = -__builtin_huge_valf( );
+            litem[r][h] = 0;
+            lsum[r][h] = lsq[r][h] = 0.f;
+        }
+    for ( int it = 0; it < nitems; it++ ) {
+        // unit-stride rows (im.es == 1): X[t + 64 i] and X[P - t - 64 i] at constant
+        // offsets from two lane pointers
+        const RowBatchItem item = load_item(items, it);
+        const auto*        ip   = as_global(static_cast<const float2*>(item.in)) + im.base(row);
+        const auto*        lo   = ip + t;
+        const auto*        hi   = ip + (P - t);
+        float2             x[E], xp[E];
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            x[i]  = load_cpx(lo, T * i);
+            xp[i] = load_cpx(hi, -T * i);
+        }
+        if ( t == 0 ) {
+            x[0].y  = 0.f;
+            xp[0].y = 0.f;
+        }
+        float2 v[E];
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const float2 e = make_float2(x[i].x + xp[i].x, x[i].y - xp[i].y);
+            const float2 d = make_float2(x[i].x - xp[i].x, x[i].y + xp[i].y);
+            const float2 c = cmulc(d, cmul(twm[N + T * i], wt));
+            v[i]           = make_float2(e.x - c.y, e.y + c.x);
+        }
+        fft.template run<+1>(v, wl, t); // v[i] = z[t + 64 i]: x[2m] = Re, x[2m+1] = Im
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            wl[FFT::Layout::slot(t + T * i)] = v[i];
+        wave_lds_fence( );
+#pragma unroll
+        for ( int r = 0; r < R; r++ ) {
+#pragma unroll
+            for ( int h = 0; h < 2; h++ ) {
+                const int    j = 2 * (t + T * r) + h;
+                const int    n = (oN + j) & (N - 1);
+                const float2 z = wl[FFT::Layout::slot(n >> 1)];
+                const float  y = ((n & 1) ? z.y : z.x) * post_scale;
+                if ( j < M ) {
+                    if ( y > lmax[r][h] ) {
+                        lmax[r][h]  = y;
+                        litem[r][h] = it;
+                    }
+                    lsum[r][h] += y;
+                    lsq[r][h] += y * y;
+                }
+            }
+        }
+        wave_lds_fence( ); // the window reads are done before the next transform writes the region
+    }
+    auto* gm = as_global(mip);
+    auto* gb = as_global(best);
+    auto* gs = as_global(sum);
+    auto* gq = as_global(sum_sq);
+#pragma unroll
+    for ( int r = 0; r < R; r++ ) {
+#pragma unroll
+        for ( int h = 0; h < 2; h++ ) {
+            const int j = 2 * (t + T * r) + h;
+            if ( j < M ) {
+                const long p = (long)row * M + j;
+                if ( gridDim.y > 1 ) {
+                    // a group's partial maps: written, not merged (an empty group leaves -inf and zeros)
+                    gm[p] = lmax[r][h];
+                    gb[p] = first_id + litem[r][h];
+                    gs[p] = lsum[r][h];
+                    gq[p] = lsq[r][h];
+                }
+                else {
+                    if ( lmax[r][h] > gm[p] ) {
+                        gm[p] = lmax[r][h];
+                        gb[p] = first_id + litem[r][h];
+                    }
+                    gs[p] = gs[p] + lsum[r][h];
+                    gq[p] = gq[p] + lsq[r][h];
+                }
+            }
+        }
+    }
+}
+
 } // namespace fastfft_amd

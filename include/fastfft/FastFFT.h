@@ -252,6 +252,16 @@ class FourierTransformer {
                       "FwdImageInvFFTBatch");
     }
 
+    // Extension (template matching): per-pixel maximum, its template id, sum and sum
+    // of squares over the batch's correlations, updated by FwdImageInvFFTBatch in
+    // place of the outputs (fastfft_set_batch_stats); all null switches it off.
+    void SetBatchStats(float* mip, int* best, float* sum, float* sum_sq, int first_id = 0) {
+        detail::check(fastfft_set_batch_stats(plan_, mip, best, sum, sum_sq, first_id), "SetBatchStats");
+    }
+
+    // Extension: mip = -inf, best = -1, sums = 0, asynchronously (fastfft_reset_batch_stats).
+    void ResetBatchStats( ) { detail::check(fastfft_reset_batch_stats(plan_), "ResetBatchStats"); }
+
     // Extension: argmax of a real [ny][pitch] device image, (x, y) and value.
     void ArgMax(const InputType* img, int nx, int ny, int pitch, int& peak_x, int& peak_y, float& value) {
         int pk[2];

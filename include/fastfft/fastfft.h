@@ -152,6 +152,28 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
 fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_to_search, fastfft_op pre_op,
                                            fastfft_op intra_op, fastfft_op post_op);
 
+/* Extension (template matching, no counterpart in the reference; for searches
+ * that keep per-pixel statistics over all templates instead of every
+ * correlation map): with statistics set, each fastfft_fwd_image_inv_batch call
+ * updates, for every position p = y * out_x + x of the out_x by out_y window
+ * (dense row-major device maps of out_x * out_y elements),
+ *   mip[p]    = max(mip[p], out_t[p]) over the batch's templates t in order,
+ *   best[p]   = first_id + t of the first template that raises mip[p] (ties keep
+ *               the earlier template, and the earlier call),
+ *   sum[p]   += out_t[p],  sum_sq[p] += out_t[p]^2,
+ * in place of writing the outputs and peaks of fastfft_set_batch (which stay
+ * required there but are not written).  Needs a batch on the batched kernels
+ * with 512- or 1024-point rows along x after any frame reduction (the
+ * describe("batch") line names the form); first_id may be changed between
+ * calls by calling this again.  All four NULL switches statistics off;
+ * fastfft_set_batch also clears them. */
+fastfft_status fastfft_set_batch_stats(fastfft_plan plan, float* mip, int* best, float* sum, float* sum_sq, int first_id);
+
+/* Extension: mip = -infinity, best = -1, sum = sum_sq = 0 over the maps set by
+ * fastfft_set_batch_stats, asynchronously on the plan's stream (a fill kernel,
+ * no memset, so it can be captured in a HIP graph). */
+fastfft_status fastfft_reset_batch_stats(fastfft_plan plan);
+
 /* Wait() (FastFFT.h:124-126): synchronise the plan's stream. */
 fastfft_status fastfft_wait(fastfft_plan plan);
 
