@@ -19,6 +19,8 @@ Beside the headline line the JSON holds:
               (fastfft_set_precision), its speedup over FP32 and its parity;
   uneven_batch 256 templates of mixed sizes (96 x 128^2, 96 x 256^2,
               64 x 384^2) in one call per step;
+  config5_fp16 256 FP16 512^2 templates against the FP16 micrograph's
+              __half2 spectrum with FP16 outputs, FP32 and split-BF16;
   stats_2dtm  the headline batch in template-matching statistics form
               (per-pixel maximum, its template, sum and sum of squares
               updated in place of the maps), against the batch's own maps;
@@ -243,6 +245,52 @@ def main() -> int:
     }
     del uouts, ubufs
 
+    # ---------------------------------------------------------------- FP16 I/O: 256 FP16 templates of 512^2
+    # (radius-220 masks; 16 distinct, each listed 16 times) against the FP16 micrograph's __half2 spectrum,
+    # FP16 512^2 outputs; FP32 and split-BF16 correlation passes
+    T5, n5 = 512, 256
+    mft16 = F.FourierTransformer("f16", "c32h", 2)
+    mft16.SetForwardFFTPlan(N, N, 1, N, N, 1)
+    mft16.SetInverseFFTPlan(N, N, 1, N, N, 1)
+    spec16 = torch.from_numpy(W.to_padded(mic.astype(np.float16), dtype=np.float16).reshape(-1)).to(dev)
+    mft16.FwdFFT(spec16, pre_op=F.Scale(1.0 / N))
+    t5 = [torch.from_numpy(W.to_padded(W.masked_template(T5, 220.0, seed=500 + i).astype(np.float16),
+                                       dtype=np.float16).reshape(-1)).to(dev) for i in range(16)]
+    c5 = {}
+    for prec in ("fp32", "split_bf16"):
+        p5 = F.FourierTransformer("f16", "c32h", 2)
+        p5.SetForwardFFTPlan(T5, T5, 1, N, N, 1)
+        p5.SetInverseFFTPlan(N, N, 1, M, M, 1)
+        p5.SetOutputWindow(-M // 2, -M // 2)
+        o5 = [torch.zeros(p5.ReturnInvOutputMemorySize(), dtype=torch.float16, device=dev) for _ in range(n5)]
+        pk5 = torch.zeros(2 * n5, dtype=torch.int32, device=dev)
+        v5 = torch.zeros(n5, dtype=torch.float32, device=dev)
+        p5.SetPrecision(prec)
+        p5.SetBatch([t5[i % 16] for i in range(n5)], [(T5, T5)] * n5, o5, pk5, v5)
+        p5.SetStream(cap)
+        post5 = F.Scale(1.0 / (N * T5 * T5))
+        g5 = capture(lambda: p5.FwdImageInvFFTBatch(spec16, intra_op=conj, post_op=post5), 1)
+        r5 = max(2, K // 10)
+        ms5 = time_graph(g5, r5, warm=3) / r5
+        p5.SetStream(torch.cuda.current_stream())
+        c5[prec] = {"value": round(n5 * 1000.0 / ms5, 1), "ms_per_call": round(ms5, 5),
+                    "form": p5.Describe("batch").splitlines()[0], "peaks": pk5.cpu().numpy()}
+        del o5
+    bytes5 = 33570816 + 524288 + 524288  # __half2 spectrum + FP16 template + FP16 output, per correlation
+    config5 = {
+        "templates": n5,
+        "workload": "256 FP16 512^2 templates (radius-220 masks) vs the FP16 4096^2 micrograph's __half2 spectrum, "
+                    "FP16 512^2 window outputs",
+        "fp32": {k: v for k, v in c5["fp32"].items() if k != "peaks"},
+        "split_bf16": {k: v for k, v in c5["split_bf16"].items() if k != "peaks"},
+        "speedup_split_bf16_vs_fp32": round(c5["fp32"]["ms_per_call"] / c5["split_bf16"]["ms_per_call"], 3),
+        "peaks_identical": bool(np.array_equal(c5["fp32"]["peaks"], c5["split_bf16"]["peaks"])),
+        "compulsory_bytes_per_corr_unamortised": bytes5,
+        "hbm_frac_unamortised_fp32": round(bytes5 * c5["fp32"]["value"] / 1e9 / HBM_PEAK_GBS, 4),
+        "timing": "HIP graph of one FwdImageInvFFTBatch call replayed; parity in tests/test_gpu_mfma.py",
+    }
+    del t5, spec16
+
     # ---------------------------------------------------------------- 3-D size change: padded forward
     # 64^3 -> 256^3 and trimmed inverse 512^3 -> 128^3 (centred), each timed alone
     def time_3d(setup):
@@ -379,6 +427,7 @@ def main() -> int:
         },
         "uneven_batch": uneven,
         "stats_2dtm": stats_2dtm,
+        "config5_fp16": config5,
         "config4_3d": config4,
         "single": {
             "value": round(1000.0 / single_ms, 1),
