@@ -209,7 +209,9 @@ def test_kat_impulse_convolution(small, big):
 # ------------------------------------------------------------------ random-data parity sweep
 FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (256, 128)), ((512, 512), (512, 512)),
              ((4096, 4096), (4096, 4096)), ((8192, 16), (8192, 16)), ((16, 8192), (32, 8192)), ((1000, 900), (1024, 1024)),
-             ((256, 256), (64, 32))]
+             ((256, 256), (64, 32)),
+             # packed R2C rows: odd non-zero length on 512-point rows, a decrease from 1024 keeping 129 outputs
+             ((301, 50), (512, 64)), ((1024, 64), (256, 32))]
 
 
 @pytest.mark.parametrize("mode", ["default", "no_packed", "generic"])
@@ -445,6 +447,29 @@ def test_fused_argmax_half_output():
     peak = torch.zeros(2, dtype=torch.int32, device=d)
     val = torch.zeros(1, dtype=torch.float32, device=d)
     ft.FwdImageInvFFTArgMax(a, img, out, peak, val)
+    rx, ry, rv = ft.ArgMax(out, nout[0], nout[1], pitch)
+    assert tuple(H.to_host(peak)) == (rx, ry) and H.to_host(val)[0] == np.float32(rv)
+
+
+@pytest.mark.parametrize("tin,n,nout,origin", [((300, 40), (1024, 64), (512, 32), (-256, 0)),
+                                               ((201, 30), (512, 64), (101, 20), (-37, 5))])
+def test_packed_rows_half_io(tin, n, nout, origin):
+    """FP16 templates and FP16 outputs through the packed 512- and 1024-point row kernels: against the FP64
+    oracle within FP16 storage rounding, and the fused peak equal to the argmax of the stored output."""
+    H.log(f"packed rows, FP16 I/O {tin} in {n} -> {nout} at {origin}")
+    torch, d, ft, a, img, dt = _argmax_case(tin, n, nout, origin, input_type="f16", seed=7)
+    # the FP16 template as stored, and the spectrum, before the call
+    tp = H.to_host(a)[: H.F.padded_pitch(tin[0]) * tin[1]].astype(np.float32).reshape(tin[1], -1)
+    spec = H.complex_from_f32(H.to_host(img), (n[0] // 2 + 1, n[1]))
+    pitch = H.F.padded_pitch(nout[0])
+    out = H.dev_zeros(ft.ReturnInvOutputMemorySize(), dt)
+    peak = torch.zeros(2, dtype=torch.int32, device=d)
+    val = torch.zeros(1, dtype=torch.float32, device=d)
+    # a post scale keeps the correlation inside the FP16 range
+    ft.FwdImageInvFFTArgMax(a, img, out, peak, val, intra_op=H.F.ConjMulThenScale(0.5), post_op=H.F.Scale(1.0 / 1024))
+    got = H.real_window(H.to_host(out).astype(np.float32), nout[0], nout[1])
+    want = O.xcorr_2d(tp, tin[0], spec, n[0], n[1], nout[0], nout[1], *origin, intra_scale=0.5, post_scale=1.0 / 1024)
+    assert np.isfinite(got).all() and O.rel_l2(got, want) < 2e-3
     rx, ry, rv = ft.ArgMax(out, nout[0], nout[1], pitch)
     assert tuple(H.to_host(peak)) == (rx, ry) and H.to_host(val)[0] == np.float32(rv)
 
