@@ -480,9 +480,9 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
     return ok( );
 }
 
-// Picks the chunk size: FASTFFT_BATCH_CHUNK if set, else the fastest of
-// 1, 2, 4, ... templates per launch on this batch (timed on a zero image), cached
-// per batch shape.  Allocates batch_dev for the chosen size and fills the descriptors.
+// Picks the chunk size: FASTFFT_BATCH_CHUNK if set, else the largest of 1, 2, 4,
+// ... templates per launch whose time on this batch (a zero image) is within 5 %
+// of the fastest, cached per batch shape.  Allocates batch_dev for the chosen size and fills the descriptors.
 static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stage>& st, CpxT img_ct) {
     const int n = (int)p->batch.size( );
     float2*   tw = nullptr;
@@ -526,9 +526,10 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
             e = hipEventCreate(&e0);
         if ( e == hipSuccess )
             e = hipEventCreate(&e1);
-        float          best = 0.f;
-        fastfft_status run  = FASTFFT_SUCCESS; // a failed launch keeps its own message
-        chosen              = 1;
+        float              best = 0.f;
+        std::vector<float> times;
+        fastfft_status     run = FASTFFT_SUCCESS; // a failed launch keeps its own message
+        chosen                 = 1;
         for ( size_t ci = 0; e == hipSuccess && ci < cand.size( ); ci++ ) {
             p->batch_chunk = cand[ci];
             if ( (run = fill_batch_items(p, st, cand[ci])) != FASTFFT_SUCCESS )
@@ -555,10 +556,17 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
             }
             if ( run )
                 break;
-            if ( e == hipSuccess && (ci == 0 || ms < best) ) {
-                best   = ms;
-                chosen = cand[ci];
-            }
+            if ( e == hipSuccess )
+                times.push_back(ms);
+        }
+        // the largest chunk within 5 % of the fastest: fewer, larger launches replay
+        // better from a graph than eager timing shows (at 64 x 384^2 -> 512^2, chunk 64
+        // replays in 286 us against 292-304 us for chunk 32, which times faster eagerly)
+        if ( ! run && e == hipSuccess && times.size( ) == cand.size( ) ) {
+            best = *std::min_element(times.begin( ), times.end( ));
+            for ( size_t ci = 0; ci < cand.size( ); ci++ )
+                if ( times[ci] <= 1.05f * best )
+                    chosen = cand[ci];
         }
         if ( e0 )
             (void)hipEventDestroy(e0);
