@@ -31,11 +31,12 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
 // One workgroup per column; columns 0 and nlines-1 share workgroup 0 when they
 // are the DC and Nyquist columns of a real transform (FASTFFT_XCORR_PACK=0
 // gives them a workgroup each, for tests).
-// Several templates per launch (io.xitems): when the columns alone give fewer
-// than about kXcorrGroupTarget workgroups, the templates are split into groups
-// along gridDim.y so that the launch still fills the GPU (each group stages its
-// column again; FASTFFT_XCORR_GROUPS overrides the count).
-constexpr int kXcorrGroupTarget = 4096;
+// Several templates per launch (io.xitems): the templates are split into groups
+// of kXcorrGroupTemplates along gridDim.y (each group stages its image column
+// again; FASTFFT_XCORR_GROUPS overrides the count).  Two templates per group
+// were fastest at 64 x 384^2 -> 512^2 on the 1024 frame: 133 us per 64-template
+// launch against 152 us with 8 per group and 142 us with 1.
+constexpr int kXcorrGroupTemplates = 2;
 
 template <int P, int Q, int RS, typename TImg>
 static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
@@ -43,7 +44,7 @@ static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* 
     const int blocks = s.nlines - pack;
     int       groups = 1;
     if ( io.xitems && io.nxitems > 1 && blocks % 8 == 0 ) {
-        groups = env_int("FASTFFT_XCORR_GROUPS", (kXcorrGroupTarget + blocks - 1) / blocks);
+        groups = env_int("FASTFFT_XCORR_GROUPS", (io.nxitems + kXcorrGroupTemplates - 1) / kXcorrGroupTemplates);
         groups = std::max(1, std::min(groups, io.nxitems));
     }
     XcorrItem one{io.src, io.dst, s.L, 0};
