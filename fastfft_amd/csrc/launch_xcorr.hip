@@ -103,14 +103,14 @@ bool xcorr_mfma_covers(const Stage& s, CpxT img_ct) {
            single_index_lines(s.gm) && s.gm.es == 1;
 }
 
-// Template groups per column: about 2048 workgroups in all, at least 4 templates
-// per group where the batch has them (the workgroup set-up is shared by its
-// group); FASTFFT_MFMA_GROUPS overrides.
+// Template groups per column of kXcorrGroupTemplates templates, as for the FP32
+// pass (at 64 x 384^2 on the 1024 frame: 326 us per 64-template launch against
+// 376 us with 16 per group); FASTFFT_MFMA_GROUPS overrides.
 static hipError_t go_xcorr_mfma(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const int n      = io.xitems ? io.nxitems : 1;
     int       groups = env_int("FASTFFT_MFMA_GROUPS", 0);
     if ( groups <= 0 )
-        groups = std::min((2048 + s.nlines - 1) / s.nlines, (n + kMfmaWaves - 1) / kMfmaWaves);
+        groups = (n + kXcorrGroupTemplates - 1) / kXcorrGroupTemplates;
     groups = std::max(1, std::min(groups, n));
     XcorrItem one{io.src, io.dst, s.L, 0};
     k_xcorr_mfma1024<<<dim3((unsigned)s.nlines, (unsigned)groups), dim3(256), 0, st>>>(
