@@ -142,6 +142,42 @@ static hipError_t try_c2r_packed(const Stage& s, const StageIO& io, const float2
     return e;
 }
 
+// Trimmed C2R passes of 2048, 4096 or 8192 points (and full-length ones asked
+// for their argmax): the packed windowed kernel, Q = N / 1024 waves per row
+// (rows_packed.hpp), with its argmax keys reduced after it.  Full-length passes
+// without argmax keep k_inv_full_wave.
+static hipError_t try_c2r_window_packed(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    if ( s.kind != StageKind::inv_c2r || ! (s.N == 2048 || s.N == 4096 || s.N == 8192) || s.M < 1 || s.M > s.N ||
+         (s.M == s.N && s.o % s.N == 0 && ! io.amax) ||
+         s.nlines < 1 || io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real ||
+         env_int("FASTFFT_NO_PACKED_ROWS", 0) || env_int("FASTFFT_NO_PACKED_WINDOW", 0) )
+        return hipErrorNotSupported;
+    const bool          keys = io.amax && s.nlines <= kArgmaxPartials;
+    unsigned long long* rk   = keys ? io.amax->keys : nullptr;
+    auto                go   = [&](auto qc) -> hipError_t {
+        constexpr int Q = decltype(qc)::value;
+        const dim3    grid((unsigned)s.nlines), block(64 * Q);
+        if ( io.dst_rt == RealT::f32 )
+            k_c2r_window_packed<512, Q, float><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+                                                                       static_cast<float*>(io.dst), to_device(s.om), s.M,
+                                                                       s.o, s.nlines, io.scale, tw, rk);
+        else
+            k_c2r_window_packed<512, Q, __half><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+                                                                        static_cast<__half*>(io.dst), to_device(s.om), s.M,
+                                                                        s.o, s.nlines, io.scale, tw, rk);
+        return hipGetLastError( );
+    };
+    hipError_t e = s.N == 2048   ? go(std::integral_constant<int, 2>{ })
+                   : s.N == 4096 ? go(std::integral_constant<int, 4>{ })
+                                 : go(std::integral_constant<int, 8>{ });
+    if ( e == hipSuccess && keys ) {
+        e = launch_argmax_keys(io.amax->keys, s.nlines, s.M, io.amax->value, io.amax->xy, st);
+        if ( e == hipSuccess && io.amax_fused )
+            *io.amax_fused = true;
+    }
+    return e;
+}
+
 // Trimmed C2R rows with P in {256, 512, 1024} and a window origin at register
 // rotation 0 or E/2: one block per row, one wave per sub-transform.
 template <int P, int Q, int RS, typename TOut>
@@ -347,6 +383,8 @@ static hipError_t try_window(const Stage& s, const StageIO& io, const float2* tw
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = try_c2r_packed(s, io, tw, st);
+        if ( e == hipErrorNotSupported )
+            e = try_c2r_window_packed(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;
     }

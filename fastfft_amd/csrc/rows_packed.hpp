@@ -334,4 +334,116 @@ __global__ void __launch_bounds__(64 * kPackedRows)
     }
 }
 
+// C2R of rows of N = 2 PP real values, PP = P Q complex points (N = 2048, 4096,
+// 8192 with P = 512), any window of M <= N outputs, one workgroup of Q waves per
+// row (the trimmed C2R x-pass of InvFFT and of the batch's frame reduction; the
+// reference's block_fft_kernel_C2R_DECREASE_XY / _NONE_XY, FastFFT.cu:1641-1692).
+// The row's even and odd outputs come from one complex PP-point inverse:
+//   Z[k] = (X[k] + conj X[PP-k]) + i W_N^{-k} (X[k] - conj X[PP-k]),  k < PP
+//   z = IFFT_PP(Z), x[2m] = Re z[m], x[2m+1] = Im z[m],
+// with Im X[0] and Im X[PP] taken as zero.  Wave q computes U_q = IFFT_P of
+// Z[Q k' + q] (decimation in time) and each output evaluates
+//   z[m] = sum_q W_PP^{-m q} U_q[m mod P]
+// by Horner's rule in W_PP^{-m}; output j < M is x[(o + j) mod N].
+template <int P, int Q, typename TOut>
+__global__ void __launch_bounds__(64 * Q)
+        k_c2r_window_packed(const float2* __restrict__ in, LineMap im, TOut* __restrict__ out, LineMap om, int M, int o,
+                            int nlines, float post_scale, const float2* __restrict__ tw,
+                            unsigned long long* __restrict__ row_keys) {
+    constexpr int T = 64, E = P / T, PP = P * Q, N = 2 * PP, NT = T * Q;
+    using FFT       = WaveFFT<P>;
+    constexpr int LDS = (Q * FFT::SLOTS > PP + 1 + PP / 32 ? Q * FFT::SLOTS : PP + 1 + PP / 32);
+    __shared__ float2 lds[LDS]; // the input row (padded), then the exchange buffers and U_q
+    __shared__ float2 s_twp[P];
+    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t    = threadIdx.x % T;
+    const int     line = xcd_row(blockIdx.x, nlines);
+    const float2* twm  = multi_twiddles(tw);
+    auto          pad  = [](int k) { return k + (k >> 5); }; // spreads the stride-Q reads over the banks
+    // ---- the row X[0 .. PP] into LDS, every load of the thread before its LDS writes
+    {
+        float2 ptv[FFT::template pass_table_rows<NT>];
+        FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+        constexpr int R  = (PP + 1 + NT - 1) / NT;
+        const auto*   lp = as_global(in) + im.base(line);
+        float2        xr[R];
+#pragma unroll
+        for ( int r = 0; r < R; r++ ) {
+            const int k = threadIdx.x + NT * r;
+            xr[r]       = k <= PP ? load_cpx(lp, (long)k * im.es) : make_float2(0.f, 0.f);
+        }
+        FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+#pragma unroll
+        for ( int r = 0; r < R; r++ ) {
+            const int k = threadIdx.x + NT * r;
+            if ( k <= PP )
+                lds[pad(k)] = xr[r];
+        }
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    // ---- Z[Q k' + q], k' = t + 64 i
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int k  = Q * (t + T * i) + q;
+        float2    xk = lds[pad(k)], xp = lds[pad(PP - k)];
+        if ( k == 0 ) {
+            xk.y = 0.f;
+            xp.y = 0.f;
+        }
+        const float2 e = make_float2(xk.x + xp.x, xk.y - xp.y);                  // X + conj Xp
+        const float2 d = make_float2(xk.x - xp.x, xk.y + xp.y);                  // X - conj Xp
+        const float2 c = cmulc(d, load_cpx(as_global(twm), N + k));              // W_N^{-k} d
+        v[i]           = make_float2(e.x - c.y, e.y + c.x);                      // e + i c
+    }
+    __syncthreads( ); // the row is read; its LDS becomes the exchange buffers
+    float2* wl = lds + q * FFT::SLOTS;
+    fft.template run<+1>(v, wl, t);
+    // U_q[m], m = t + 64 i, in natural order in this wave's region
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        wl[t + T * i] = v[i];
+    __syncthreads( );
+    // ---- the window
+    const int          oN   = ((o % N) + N) % N;
+    auto*              op   = as_global(out) + om.base(line);
+    unsigned long long best = 0;
+    for ( int j = threadIdx.x; j < M; j += NT ) {
+        const int    n  = (oN + j) & (N - 1);
+        const int    m  = n >> 1;
+        const int    mm = m & (P - 1);
+        const float2 w  = load_cpx(as_global(twm), PP + m); // W_PP^{m}; the inverse takes its conjugate
+        float2       z  = lds[(Q - 1) * FFT::SLOTS + mm];
+#pragma unroll
+        for ( int qq = Q - 2; qq >= 0; qq-- )
+            z = cadd(cmulc(z, w), lds[qq * FFT::SLOTS + mm]);
+        const float y = ((n & 1) ? z.y : z.x) * post_scale;
+        store_real(op, (long)j * om.es, y);
+        if ( row_keys ) {
+            // the value as stored (rounded when the output is half precision)
+            const auto k = argmax_key(convert_real<TOut>(y), (unsigned)(line * M + j));
+            best         = k > best ? k : best;
+        }
+    }
+    if ( row_keys ) { // the same for every thread of the block
+        __shared__ unsigned long long s_key[Q];
+#pragma unroll
+        for ( int off = 32; off > 0; off >>= 1 ) {
+            const unsigned long long other = __shfl_xor(best, off);
+            best                           = other > best ? other : best;
+        }
+        if ( t == 0 )
+            s_key[q] = best;
+        __syncthreads( );
+        if ( threadIdx.x == 0 ) {
+#pragma unroll
+            for ( int qq = 1; qq < Q; qq++ )
+                best = s_key[qq] > best ? s_key[qq] : best;
+            as_global(row_keys)[line] = best;
+        }
+    }
+}
+
 } // namespace fastfft_amd

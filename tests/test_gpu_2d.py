@@ -238,7 +238,9 @@ INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256,
              ((1024, 1024), (1000, 1000), (3, -5)),
              # windowed inverse rows: M > P = 512, origins off the 64-lane grid, non-square windows
              ((4096, 4096), (1024, 1024), (-256, -300)), ((2048, 2048), (700, 96), (13, -77)),
-             ((512, 256), (300, 256), (-7, 0))]
+             ((512, 256), (300, 256), (-7, 0)),
+             # 8192-point C2R rows cut to a window off the lane grid
+             ((8192, 64), (3000, 64), (-1000, 0))]
 
 
 @pytest.mark.parametrize("mode", ["default", "no_window", "no_packed", "generic"])
@@ -397,7 +399,10 @@ def test_xcorr_4096_384_trim_512_centred():
 ARGMAX_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((40, 200), (64, 2048), (24, 256), (-12, -128)),
                 ((100, 60), (256, 128), (100, 60), (-50, -30)), ((24, 1000), (32, 4096), (16, 1024), (0, -512)),
                 # 1024- and 512-point rows: the packed C2R kernel with its per-row keys, odd origins
-                ((300, 40), (1024, 64), (512, 32), (-256, 0)), ((200, 30), (512, 64), (101, 20), (-37, 5))]
+                ((300, 40), (1024, 64), (512, 32), (-256, 0)), ((200, 30), (512, 64), (101, 20), (-37, 5)),
+                # 2048- to 8192-point rows: the packed windowed C2R kernel with its per-row keys
+                ((300, 24), (4096, 32), (512, 32), (-256, 0)), ((64, 30), (2048, 64), (777, 50), (13, -5)),
+                ((100, 12), (8192, 16), (2000, 16), (-1000, 0))]
 
 
 def _argmax_case(tin, n, nout, origin, input_type="f32", seed=0):
