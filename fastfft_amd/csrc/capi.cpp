@@ -711,11 +711,32 @@ static fastfft_status run_subframe_batch(fastfft_plan p, const void* image, floa
                                          const float2* tw, unsigned parts = kBatchAll) {
     const float frame_scale = 1.f / ((float)p->sub_fx * (float)p->sub_fy);
     if ( parts & kBatchFrame ) {
-        if ( auto rc = run_stage_list(p, p->sub_inv, "batch frame", const_cast<void*>(image), p->sub_s, nullptr, 1.f, 1.f,
-                                      frame_scale, 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+        const auto &inv = p->sub_inv, &fwd = p->sub_fwd;
+        // the inverse's C2R rows and the forward's R2C rows are the frame's rows: one kernel
+        // keeps them in LDS (the real frame is not stored)
+        const bool fuse = inv.size( ) >= 2 && fwd.size( ) >= 2 && inv.back( ).src_elem == Elem::cpx_f32 &&
+                          inv.back( ).dst == Buf::user_out && fwd.front( ).src == Buf::user_in &&
+                          fwd.front( ).dst_elem == Elem::cpx_f32 &&
+                          (inv.back( ).src == Buf::ws_a || inv.back( ).src == Buf::ws_b) &&
+                          (fwd.front( ).dst == Buf::ws_a || fwd.front( ).dst == Buf::ws_b) &&
+                          inv.back( ).src != fwd.front( ).dst && c2r_r2c_rows_covers(inv.back( ), fwd.front( ));
+        const int ilast = (int)inv.size( ) - 1 - (fuse ? 1 : 0);
+        if ( auto rc = run_stage_list(p, inv, "batch frame", const_cast<void*>(image), p->sub_s, nullptr, 1.f, 1.f,
+                                      frame_scale, 0, ilast, nullptr, nullptr, p->sub_wa, p->sub_wb) )
             return rc;
-        if ( auto rc = run_stage_list(p, p->sub_fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f, 0,
-                                      -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+        if ( fuse ) {
+            // the C2R keeps the list's post scale (its last stage), the R2C the forward's pre scale 1
+            float2*          tw  = nullptr;
+            if ( auto rc = ensure_device_state(p, &tw) )
+                return rc;
+            const auto*      src = inv.back( ).src == Buf::ws_a ? p->sub_wa : p->sub_wb;
+            float2*          dst = fwd.front( ).dst == Buf::ws_a ? p->sub_wa : p->sub_wb;
+            const hipError_t e   = launch_c2r_r2c_rows(inv.back( ), fwd.front( ), src, dst, frame_scale, 1.f, tw, p->stream);
+            if ( e != hipSuccess )
+                return hip_fail(e, "batch frame rows");
+        }
+        if ( auto rc = run_stage_list(p, fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f,
+                                      fuse ? 1 : 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
             return rc;
     }
     return run_batch_chunks(p, p->sub_xc, p->sub_h, CpxT::c64, pre, intra, post, tw, parts);

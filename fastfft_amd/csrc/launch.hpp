@@ -84,6 +84,14 @@ hipError_t launch_stats_reset(const BatchStats& bs, long n, hipStream_t stream);
 
 hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 hipError_t launch_inv(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
+
+// C2R stage c (2048-8192 points, M <= 1024 outputs) and the 1024-point R2C stage f
+// of the same rows in one kernel (the image's reduction to a correlation frame and
+// the frame's forward rows): src is c's input, dst f's output, the real rows are
+// not stored.  hipErrorNotSupported when the pair does not fit.
+bool       c2r_r2c_rows_covers(const Stage& c, const Stage& f);
+hipError_t launch_c2r_r2c_rows(const Stage& c, const Stage& f, const float2* src, float2* dst, float post, float pre,
+                               const float2* tw, hipStream_t stream);
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
 
 // Entries of the twiddle table: W_8192^k for k < 8192, then the

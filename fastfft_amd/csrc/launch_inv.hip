@@ -178,6 +178,35 @@ static hipError_t try_c2r_window_packed(const Stage& s, const StageIO& io, const
     return e;
 }
 
+// A C2R pass of 2048, 4096 or 8192 points cut to M <= 1024 outputs whose rows are
+// the rows of a 1024-point R2C pass (k_c2r_r2c_rows): the real rows stay in LDS.
+static bool same_map(const HostLineMap& a, const HostLineMap& b) {
+    return a.s1 == b.s1 && a.s2 == b.s2 && a.es == b.es && a.n2 == b.n2;
+}
+
+bool c2r_r2c_rows_covers(const Stage& c, const Stage& f) {
+    return c.kind == StageKind::inv_c2r && (c.N == 2048 || c.N == 4096 || c.N == 8192) && c.M >= 1 && c.M <= 1024 &&
+           c.nlines >= 1 && f.kind == StageKind::fwd_real && f.N == 1024 && f.L == c.M && f.nlines == c.nlines &&
+           f.nout >= 1 && f.nout <= 513 && same_map(c.om, f.im) && ! env_int("FASTFFT_NO_FRAME_FUSION", 0) &&
+           ! env_int("FASTFFT_NO_PACKED_ROWS", 0) && ! env_int("FASTFFT_NO_PACKED_WINDOW", 0) &&
+           ! env_int("FASTFFT_NO_WAVE_ROWS", 0);
+}
+
+hipError_t launch_c2r_r2c_rows(const Stage& c, const Stage& f, const float2* src, float2* dst, float post, float pre,
+                               const float2* tw, hipStream_t st) {
+    if ( ! c2r_r2c_rows_covers(c, f) )
+        return hipErrorNotSupported;
+    auto go = [&](auto qc) -> hipError_t {
+        constexpr int Q = decltype(qc)::value;
+        k_c2r_r2c_rows<Q><<<dim3((unsigned)c.nlines), dim3(64 * Q), 0, st>>>(src, to_device(c.im), c.M, c.o, c.nlines, post,
+                                                                             dst, to_device(f.om), f.nout, pre, tw);
+        return hipGetLastError( );
+    };
+    return c.N == 2048 ? go(std::integral_constant<int, 2>{ })
+           : c.N == 4096 ? go(std::integral_constant<int, 4>{ })
+                         : go(std::integral_constant<int, 8>{ });
+}
+
 // Trimmed C2R rows with P in {256, 512, 1024} and a window origin at register
 // rotation 0 or E/2: one block per row, one wave per sub-transform.
 template <int P, int Q, int RS, typename TOut>

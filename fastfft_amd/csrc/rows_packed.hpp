@@ -446,4 +446,118 @@ __global__ void __launch_bounds__(64 * Q)
     }
 }
 
+// A trimmed C2R row followed by the R2C of the same row (no counterpart in the
+// reference; the image's reduction to a 1024-point-wide correlation frame and
+// that frame's forward rows): one workgroup of Q waves per line computes the
+// M <= 1024 window outputs of the N = 1024 Q-point C2R exactly as
+// k_c2r_window_packed does (times post_scale), keeps them in LDS instead of
+// storing them, and wave 0 then runs the 1024-point packed R2C of
+// k_r2c_rows_packed on them (times pre_scale, zero past M), storing the k < nout
+// spectrum values at om2.  Outputs equal those of the two kernels run in turn,
+// without the real row's round trip through HBM.
+template <int Q>
+__global__ void __launch_bounds__(64 * Q)
+        k_c2r_r2c_rows(const float2* __restrict__ in, LineMap im, int M, int o, int nlines, float post_scale,
+                       float2* __restrict__ out, LineMap om2, int nout, float pre_scale, const float2* __restrict__ tw) {
+    constexpr int P = 512, T = 64, E = P / T, PP = P * Q, N = 2 * PP, NT = T * Q, N2 = 2 * P;
+    using FFT       = WaveFFT<P>;
+    constexpr int LDS = (Q * FFT::SLOTS > PP + 1 + PP / 32 ? Q * FFT::SLOTS : PP + 1 + PP / 32);
+    __shared__ float2 lds[LDS]; // the input row (padded), then the exchange buffers and U_q
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_row[P]; // the real window row, x[2m] + i x[2m+1]
+    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     t    = threadIdx.x % T;
+    const int     line = xcd_row(blockIdx.x, nlines);
+    const float2* twm  = multi_twiddles(tw);
+    auto          pad  = [](int k) { return k + (k >> 5); };
+    {
+        float2 ptv[FFT::template pass_table_rows<NT>];
+        FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+        constexpr int R  = (PP + 1 + NT - 1) / NT;
+        const auto*   lp = as_global(in) + im.base(line);
+        float2        xr[R];
+#pragma unroll
+        for ( int r = 0; r < R; r++ ) {
+            const int k = threadIdx.x + NT * r;
+            xr[r]       = k <= PP ? load_cpx(lp, (long)k * im.es) : make_float2(0.f, 0.f);
+        }
+        FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+#pragma unroll
+        for ( int r = 0; r < R; r++ ) {
+            const int k = threadIdx.x + NT * r;
+            if ( k <= PP )
+                lds[pad(k)] = xr[r];
+        }
+    }
+    __syncthreads( );
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int k  = Q * (t + T * i) + q;
+        float2    xk = lds[pad(k)], xp = lds[pad(PP - k)];
+        if ( k == 0 ) {
+            xk.y = 0.f;
+            xp.y = 0.f;
+        }
+        const float2 e = make_float2(xk.x + xp.x, xk.y - xp.y);     // X + conj Xp
+        const float2 d = make_float2(xk.x - xp.x, xk.y + xp.y);     // X - conj Xp
+        const float2 c = cmulc(d, load_cpx(as_global(twm), N + k)); // W_N^{-k} d
+        v[i]           = make_float2(e.x - c.y, e.y + c.x);         // e + i c
+    }
+    __syncthreads( );
+    float2* wl = lds + q * FFT::SLOTS;
+    fft.template run<+1>(v, wl, t);
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        wl[t + T * i] = v[i];
+    __syncthreads( );
+    // ---- the window row, zero past M, into s_row
+    const int oN = ((o % N) + N) % N;
+    float*    xr = reinterpret_cast<float*>(s_row);
+    for ( int j = threadIdx.x; j < N2; j += NT ) {
+        float y = 0.f;
+        if ( j < M ) {
+            const int    n  = (oN + j) & (N - 1);
+            const int    m  = n >> 1;
+            const int    mm = m & (P - 1);
+            const float2 w  = load_cpx(as_global(twm), PP + m);
+            float2       z  = lds[(Q - 1) * FFT::SLOTS + mm];
+#pragma unroll
+            for ( int qq = Q - 2; qq >= 0; qq-- )
+                z = cadd(cmulc(z, w), lds[qq * FFT::SLOTS + mm]);
+            y = ((n & 1) ? z.y : z.x) * post_scale;
+        }
+        xr[j] = y;
+    }
+    __syncthreads( ); // U_q are read; wave 0 reuses its region for the R2C
+    if ( q != 0 )
+        return;
+    const float2 wt = load_cpx(as_global(twm), N2 + t); // W_N2^t
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const float2 a = s_row[t + T * i];
+        v[i]           = make_float2(a.x * pre_scale, a.y * pre_scale);
+    }
+    fft.template run<-1>(v, lds, t); // v[i] = Z[t + 64 i]
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        lds[FFT::Layout::slot(t + T * i)] = v[i];
+    wave_lds_fence( );
+    auto* op = as_global(out) + om2.base(line);
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int    k  = t + T * i;
+        const float2 zp = lds[FFT::Layout::slot((P - k) & (P - 1))];
+        const float2 a  = make_float2(0.5f * (v[i].x + zp.x), 0.5f * (v[i].y - zp.y)); // (Z + conj Zp) / 2
+        const float2 b  = make_float2(0.5f * (v[i].x - zp.x), 0.5f * (v[i].y + zp.y)); // (Z - conj Zp) / 2
+        const float2 c  = cmul(cmul(twm[N2 + T * i], wt), b);                          // W_N2^k b
+        if ( k < nout )
+            store_cpx(op, (long)k * om2.es, make_float2(a.x + c.y, a.y - c.x)); // a - i c
+    }
+    if ( t == 0 && P < nout )
+        store_cpx(op, (long)P * om2.es, make_float2(v[0].x - v[0].y, 0.f));
+}
+
 } // namespace fastfft_amd
