@@ -44,11 +44,34 @@ __global__ void __launch_bounds__(64 * Q)
         constexpr int KN = HERM ? H + 1 : N;
         constexpr int MR = (KN + 2 * NT - 1) / (2 * NT);
         float2        u[MR][2];
+        // unit-stride FP32 lines on 16-byte boundaries: each lane's pair in one load
+        bool pairs = false;
+        if constexpr ( std::is_same_v<TIn, float2> )
+            pairs = im.es == 1 && (im.base(line) & 1) == 0 && (reinterpret_cast<unsigned long>(in) & 15) == 0;
+        if ( pairs ) {
+            if constexpr ( std::is_same_v<TIn, float2> ) {
 #pragma unroll
-        for ( int r = 0; r < MR; r++ ) {
-            const int m = threadIdx.x + NT * r;
-            u[r][0]     = 2 * m < KN ? load_cpx(lp, (long)(2 * m) * im.es) : make_float2(0.f, 0.f);
-            u[r][1]     = 2 * m + 1 < KN ? load_cpx(lp, (long)(2 * m + 1) * im.es) : make_float2(0.f, 0.f);
+                for ( int r = 0; r < MR; r++ ) {
+                    const int m = threadIdx.x + NT * r;
+                    if ( 2 * m + 1 < KN ) {
+                        const float4 a = load_cpx_pair(lp, m);
+                        u[r][0]        = make_float2(a.x, a.y);
+                        u[r][1]        = make_float2(a.z, a.w);
+                    }
+                    else {
+                        u[r][0] = 2 * m < KN ? load_cpx(lp, (long)(2 * m)) : make_float2(0.f, 0.f);
+                        u[r][1] = make_float2(0.f, 0.f);
+                    }
+                }
+            }
+        }
+        else {
+#pragma unroll
+            for ( int r = 0; r < MR; r++ ) {
+                const int m = threadIdx.x + NT * r;
+                u[r][0]     = 2 * m < KN ? load_cpx(lp, (long)(2 * m) * im.es) : make_float2(0.f, 0.f);
+                u[r][1]     = 2 * m + 1 < KN ? load_cpx(lp, (long)(2 * m + 1) * im.es) : make_float2(0.f, 0.f);
+            }
         }
         tws.store(s_twp, s_twn);
         s_w1[threadIdx.x] = w1;
