@@ -111,7 +111,8 @@ __global__ void __launch_bounds__(kArgmaxThreads)
     rv += blockIdx.x;
     rxy += 2 * blockIdx.x;
     unsigned long long            best = 0;
-    for ( int i = threadIdx.x; i < n; i += kArgmaxThreads ) {
+#pragma unroll 4
+    for ( int i = threadIdx.x; i < n; i += kArgmaxThreads ) { // the loads of four iterations in flight together
         const unsigned long long k = keys[i];
         best                       = k > best ? k : best;
     }
