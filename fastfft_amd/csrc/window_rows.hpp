@@ -9,8 +9,10 @@
 // evaluated per output by Horner's rule in w = W_N^{-n}:
 //   y[n] = U_0 + w (U_1 + w (U_2 + ... + w U_{Q-1})).
 // U_q is kept in LDS in natural order, so an output reads it at any rotation
-// (consecutive outputs read consecutive words: no bank conflicts), and one
-// twiddle per output comes from W_N^{o} W_N^{tid} W_N^{NT r}, output j = tid + NT r.
+// (consecutive outputs read consecutive words: no bank conflicts), and the
+// twiddle W_N^{o + j} of output j comes from the table.  The workgroup's LDS is
+// the exchange buffers and the pass table only (36 KB at N = 4096: four
+// workgroups per CU).
 // HERM: the line holds X[0 .. N/2] of a Hermitian sequence (X[N-k] = conj X[k]
 // above) and only the real part of y is stored (REAL_OUT).
 #pragma once
@@ -28,8 +30,6 @@ __global__ void __launch_bounds__(64 * Q)
     static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
     __shared__ float2 lds[Q * FFT::SLOTS];
     __shared__ float2 s_twp[P];
-    __shared__ float2 s_twn[NT];
-    __shared__ float2 s_w1[NT]; // W_N^k, k < NT
     const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int     t    = threadIdx.x % T;
     const int     line = xcd_row(blockIdx.x, nlines);
@@ -37,9 +37,8 @@ __global__ void __launch_bounds__(64 * Q)
     // ---- the input line into LDS, decimated by Q, two values per lane, every
     // load of the thread in flight before its LDS writes
     {
-        RowTwiddleStage<P, Q> tws;
-        tws.load(twm);
-        const float2  w1 = load_cpx(as_global(twm), N + threadIdx.x);
+        float2 ptv[FFT::template pass_table_rows<NT>];
+        FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
         const auto*   lp = as_global(in) + im.base(line);
         constexpr int KN = HERM ? H + 1 : N;
         constexpr int MR = (KN + 2 * NT - 1) / (2 * NT);
@@ -73,8 +72,7 @@ __global__ void __launch_bounds__(64 * Q)
                 u[r][1]     = 2 * m + 1 < KN ? load_cpx(lp, (long)(2 * m + 1) * im.es) : make_float2(0.f, 0.f);
             }
         }
-        tws.store(s_twp, s_twn);
-        s_w1[threadIdx.x] = w1;
+        FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
 #pragma unroll
         for ( int r = 0; r < MR; r++ ) {
             const int m = threadIdx.x + NT * r;
@@ -109,17 +107,14 @@ __global__ void __launch_bounds__(64 * Q)
         wl[t + T * i] = v[i];
     __syncthreads( );
     // ---- the window: Horner over q for each output
-    const int    oN    = ((o % N) + N) % N;
-    const int    oP    = oN & (P - 1);
-    const float2 w_o   = twm[N + oN];          // W_N^{o}
-    const float2 w_tid = s_w1[threadIdx.x];   // W_N^{tid}
-    TOut*        op    = out + om.base(line);
+    const int oN = ((o % N) + N) % N;
+    const int oP = oN & (P - 1);
+    TOut*     op = out + om.base(line);
     for ( int r = 0; r * NT < M; r++ ) {
         const int j = (int)threadIdx.x + NT * r;
         if ( j >= M )
             break;
-        const float2 wr = twm[N + ((NT * r) & (N - 1))]; // wave-uniform
-        const float2 w  = cmul(cmul(w_o, w_tid), wr);     // W_N^{o + j}
+        const float2 w  = load_cpx(as_global(twm), N + ((oN + j) & (N - 1))); // W_N^{o + j}
         const int    m  = (oP + j) & (P - 1);
         float2       acc = lds[(Q - 1) * FFT::SLOTS + m];
 #pragma unroll
