@@ -457,10 +457,13 @@ def test_fused_argmax_half_output():
 
 
 @pytest.mark.parametrize("tin,n,nout,origin", [((300, 40), (1024, 64), (512, 32), (-256, 0)),
-                                               ((201, 30), (512, 64), (101, 20), (-37, 5))])
+                                               ((201, 30), (512, 64), (101, 20), (-37, 5)),
+                                               ((300, 24), (4096, 32), (512, 32), (-256, 0)),
+                                               ((64, 12), (8192, 16), (2000, 16), (-1000, 0))])
 def test_packed_rows_half_io(tin, n, nout, origin):
-    """FP16 templates and FP16 outputs through the packed 512- and 1024-point row kernels: against the FP64
-    oracle within FP16 storage rounding, and the fused peak equal to the argmax of the stored output."""
+    """FP16 templates and FP16 outputs through the packed 512- and 1024-point row kernels and the packed
+    windowed C2R of 4096 and 8192 points: against the FP64 oracle within FP16 storage rounding, and the
+    fused peak equal to the argmax of the stored output."""
     H.log(f"packed rows, FP16 I/O {tin} in {n} -> {nout} at {origin}")
     torch, d, ft, a, img, dt = _argmax_case(tin, n, nout, origin, input_type="f16", seed=7)
     # the FP16 template as stored, and the spectrum, before the call
