@@ -48,10 +48,14 @@ def _setup(sizes, n, nout, origin, seed=0):
 BATCHES = [([(128, 128), (256, 256), (384, 384), (128, 128), (384, 384)], (4096, 4096), (512, 512), (-256, -256)),
            ([(200, 200), (160, 120), (97, 200)], (2048, 2048), (150, 200), (-70, 30)),
            ([(40, 30), (64, 64), (17, 50)], (256, 128), (64, 32), (-32, -16)),
-           ([(100, 60), (100, 60)], (256, 128), (100, 60), (-50, -30))]
+           ([(100, 60), (100, 60)], (256, 128), (100, 60), (-50, -30)),
+           # 1024-wide frames of 2048- and 8192-point images: the frame's fused C2R + R2C rows at Q = 2 and 8
+           ([(256, 256), (200, 180)], (2048, 2048), (512, 400), (-256, -200)),
+           ([(300, 200), (128, 128)], (8192, 1024), (512, 64), (-256, -32))]
 
 # frame of the batches that run on a reduced frame
-FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "512x512"}
+FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "512x512",
+          ((2048, 2048), (512, 400)): "1024x1024", ((8192, 1024), (512, 64)): "1024x512"}
 
 
 @pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "full_frame", "per_template"])
