@@ -33,9 +33,13 @@ Beside the headline line the JSON holds:
               replays of that part alone (fastfft_run_batch_stage), on the
               stream the graph runs on; traffic from the rocprofv3 counter
               summary committed under profiles/;
-  cpu_baseline the FP64 oracle (oracle/, "port") on a bounded sample of the
-              same correlations on this host's cores; its first correlation
-              also gives the parity figures.
+  cpu_baseline the same correlation on this host's cores through the FFTW3
+              single-precision API (FFTW if installed, else Intel MKL's FFTW3
+              interface, else scipy.fft; oracle/cpu_baseline.c): micrograph
+              r2c once, then per template pad, r2c, image * conj, c2r, crop,
+              argmax, on a bounded sample of template 0's correlations; the
+              library, its version, the thread count and the CPU model are
+              reported.  The FP64 oracle (oracle/) gives the parity figures.
 """
 from __future__ import annotations
 
@@ -52,6 +56,11 @@ if ROOT not in sys.path:
 
 N, T, M = 4096, 384, 512
 HBM_PEAK_GBS = 8000.0
+# the split-BF16 correlation pass of the 1024-point frame: per column a Toeplitz product of 512 output
+# rows by K = 2 x 384 interleaved template values for every template, as 2 (real and imaginary part)
+# x 3 (hi*hi, hi*lo, lo*hi) products; v_mfma_f32_32x32x16_bf16 instructions per template column
+MFMA_KERNEL = "k_xcorr_toeplitz"
+MFMA_PER_COLUMN = 2 * 3 * (M // 32) * (2 * T // 16) // 32
 TRAFFIC_PROFILE = "profiles/xcorr_col_frame1024_batch_hbm_traffic.md"
 
 
@@ -61,7 +70,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200, help="timed steps (batched calls)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64, help="templates per step")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-oracle sample length")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-batch", type=int, default=10, help="correlations per captured graph (single form)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -218,6 +227,15 @@ def main() -> int:
     sft, souts, spk, s_ms, _ = batch_form("split_bf16")
     s_out0 = souts[0].cpu().numpy()
     s_peaks_same = bool(np.array_equal(spk.cpu().numpy(), b_pk_all))
+    s_head = sft.Describe("batch").splitlines()[0]
+    s_chunk = int(re.search(r"chunk=(\d+)", s_head).group(1))
+    g_sx = capture(lambda: sft.RunBatchStage(2, spec, intra_op=conj), 1)
+    s_xc_ms = time_graph(g_sx, reps) / reps / ((B + s_chunk - 1) // s_chunk)
+    # matrix-core work of the split-BF16 correlation pass per launch: MFMA_PER_COLUMN
+    # v_mfma_f32_32x32x16_bf16 (32 x 32 x 16 x 2 flops each) per template column, over the
+    # fx / 2 column workgroups (the DC and Nyquist columns share one) and whole groups of 32 templates
+    s_mfma_flops = (s_chunk + 31) // 32 * 32 * (wf - 1) * MFMA_PER_COLUMN * 32 * 32 * 16 * 2
+    s_mfma_tflops = s_mfma_flops / (s_xc_ms * 1e-3) / 1e12
     sft.SetStream(torch.cuda.current_stream())
 
     # ---------------------------------------------------------------- uneven batch of 256 (96 x 128^2,
@@ -276,7 +294,8 @@ def main() -> int:
         c5[prec] = {"value": round(n5 * 1000.0 / ms5, 1), "ms_per_call": round(ms5, 5),
                     "form": p5.Describe("batch").splitlines()[0], "peaks": pk5.cpu().numpy()}
         del o5
-    bytes5 = 33570816 + 524288 + 524288  # __half2 spectrum + FP16 template + FP16 output, per correlation
+    # __half2 spectrum read once per call, each FP16 template in and each FP16 window out once
+    bytes5_call = 33570816 + n5 * (524288 + 524288)
     config5 = {
         "templates": n5,
         "workload": "256 FP16 512^2 templates (radius-220 masks) vs the FP16 4096^2 micrograph's __half2 spectrum, "
@@ -285,8 +304,8 @@ def main() -> int:
         "split_bf16": {k: v for k, v in c5["split_bf16"].items() if k != "peaks"},
         "speedup_split_bf16_vs_fp32": round(c5["fp32"]["ms_per_call"] / c5["split_bf16"]["ms_per_call"], 3),
         "peaks_identical": bool(np.array_equal(c5["fp32"]["peaks"], c5["split_bf16"]["peaks"])),
-        "compulsory_bytes_per_corr_unamortised": bytes5,
-        "hbm_frac_unamortised_fp32": round(bytes5 * c5["fp32"]["value"] / 1e9 / HBM_PEAK_GBS, 4),
+        "compulsory_bytes_per_call": bytes5_call,
+        "hbm_frac_per_call_fp32": round(bytes5_call / (c5["fp32"]["ms_per_call"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "timing": "HIP graph of one FwdImageInvFFTBatch call replayed; parity in tests/test_gpu_mfma.py",
     }
     del t5, spec16
@@ -371,9 +390,17 @@ def main() -> int:
         torch.cuda.synchronize()
         del ca, cb
 
-    # compulsory bytes of one correlation as a separate job: template in, spectrum in, 512^2 output out
-    corr_bytes = 4 * T * T + 8 * w * N + 4 * M * M
+    # compulsory bytes of one step counted once per batch: the image spectrum read once, each template
+    # read once and each 512^2 window written once
+    step_bytes = 8 * w * N + B * (4 * T * T + 4 * M * M)
     value = B * 1000.0 / ms_step
+    # FP32 arithmetic of the correlation pass per launch, counted as 5 n log2 n flops per n-point complex
+    # transform (a forward and an inverse transform of fy points per template column) plus 6 flops per
+    # point for the image product; one VALU lane does one FP32 FMA (2 flops) per cycle, so the
+    # non-packed ceiling is half of the 157.3 TFLOP/s packed figure
+    fft_flops = lambda n: 5.0 * n * np.log2(n)
+    xc_flops = chunk * wf * (2 * fft_flops(fy) + 6.0 * fy)
+    valu_tflops = xc_flops / (xc_ms * 1e-3) / 1e12
     result = {
         "metric": "XCorr/s 4096^2 vs 384^2 tmpl, trim 512^2",
         "value": round(value, 1),
@@ -403,16 +430,29 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": TRAFFIC_PROFILE if traffic else None,
+            "traffic_over_algorithmic": round(traffic / xc_bytes, 3) if traffic else None,
+            "measured_hbm_frac": round(traffic / (xc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "algorithmic_bytes_per_launch": xc_bytes,
             "avg_launch_ms": round(xc_ms, 5),
             "launches_per_step": n_launch,
+            "valu": {
+                "flops_per_launch": round(xc_flops),
+                "achieved_tflops": round(valu_tflops, 2),
+                "peak_nonpacked_tflops": 78.6,
+                "frac_nonpacked": round(valu_tflops / 78.6, 4),
+                "peak_packed_tflops": 157.3,
+                "counting": "5 n log2 n per n-point complex transform (forward + inverse per template column) "
+                            "+ 6 per point for the image product",
+            },
         },
         "step": {
             "form": head,
             "part_ms": {k: round(v, 5) for k, v in part_ms.items()},
             "set_batch_s": round(set_s, 3),
-            "compulsory_bytes_per_corr_unamortised": corr_bytes,
-            "hbm_frac_unamortised": round(corr_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
+            "compulsory_bytes_per_step": step_bytes,
+            "hbm_frac_per_step": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "compulsory_bytes_counting": "image spectrum read once per call + per template its 384^2 input "
+                                         "and its 512^2 window output",
             "timing": f"HIP graph of one FwdImageInvFFTBatch call replayed {K} times after {args.warmup} warm-up "
                       "replays; part_ms from graphs of one part",
             "micrograph_fwd_ms": round(mic_fwd_ms, 4),
@@ -424,6 +464,16 @@ def main() -> int:
             "ms_per_step": round(s_ms, 5),
             "speedup_vs_fp32": round(ms_step / s_ms, 3),
             "peaks_identical_to_fp32": s_peaks_same,
+            "form": s_head,
+            "correlation_pass": {
+                "kernel": MFMA_KERNEL,
+                "avg_launch_ms": round(s_xc_ms, 5),
+                "mfma_flops_per_launch": s_mfma_flops,
+                "achieved_tflops": round(s_mfma_tflops, 1),
+                "peak_bf16_dense_tflops": 2500.0,
+                "mfma_frac": round(s_mfma_tflops / 2500.0, 4),
+                "speedup_vs_fp32_pass": round(xc_ms / s_xc_ms, 3),
+            },
         },
         "uneven_batch": uneven,
         "stats_2dtm": stats_2dtm,
@@ -441,27 +491,30 @@ def main() -> int:
     }
 
     if not args.no_cpu_baseline:
+        from oracle import cpu_baseline as CB
         from oracle import oracle as O
 
-        spec_h = spec.cpu().numpy()[: 2 * w * N].view(np.complex64).reshape(w, N).astype(np.complex128)
-        tp = W.to_padded(tmpl)
-        n_done, t0 = 0, time.perf_counter()
-        want = None
-        while True:
-            o = O.xcorr_2d(tp, T, spec_h, N, N, M, M, -M // 2, -M // 2)
-            want = o if want is None else want
-            n_done += 1
-            if time.perf_counter() - t0 >= args.cpu_seconds and n_done >= 2:
-                break
-        dt = time.perf_counter() - t0
+        # CPU baseline: the same correlation (template 0 against the micrograph, 512^2 window, argmax)
+        # through the FFTW3 API on this host's cores, a bounded sample timed after two warm-up runs
+        cb = CB.xcorr(mic, tmpl, M, warmup=2, min_corr=10, max_seconds=args.cpu_seconds)
         result["cpu_baseline"] = {
-            "value": round(n_done / dt, 3),
+            "value": round(cb["corr_per_s"], 3),
             "unit": "correlations/s",
-            "cores": O.num_threads(),
+            "cores": cb["threads"],
             "kind": "port",
-            "sample": f"{n_done} full correlations (template 0 against the GPU micrograph spectrum, FP64 oracle, "
-                      f"{O.num_threads()} OpenMP threads) in {dt:.1f} s",
+            "library": cb["library"],
+            "cpu_model": cb["cpu_model"],
+            "micrograph_fwd_ms": round(cb["mic_fwd_s"] * 1e3, 3),
+            "peak_matches_gpu": [int(v) for v in cb["peak_xy"]] == b_pk,
+            "sample": f"{cb['n_done']} FP32 correlations of template 0 (384^2 zero padded to 4096^2, r2c, "
+                      f"image * conj, c2r, 512^2 crop, argmax) in {cb['corr_s']:.2f} s on {cb['threads']} threads, "
+                      "after 2 untimed ones; the micrograph r2c is timed apart",
+            "gpu_over_cpu": round(value / cb["corr_per_s"], 1) if cb["corr_per_s"] > 0 else None,
         }
+
+        # parity: the FP64 oracle of the same correlation
+        spec_h = spec.cpu().numpy()[: 2 * w * N].view(np.complex64).reshape(w, N).astype(np.complex128)
+        want = O.xcorr_2d(W.to_padded(tmpl), T, spec_h, N, N, M, M, -M // 2, -M // 2)
         wx, wy, _ = O.argmax_2d(want)
 
         def window(a):
@@ -473,6 +526,7 @@ def main() -> int:
             "max_abs_ref": float(f"{np.abs(want).max():.3e}"),
             "argmax_match": [wx, wy] == b_pk,
             "tolerance": 2e-6,
+            "cpu_baseline_rel_l2_vs_fp64": float(f"{O.rel_l2(cb['window'].astype(np.float64), want):.3e}"),
         }
         result["split_bf16"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(s_out0), want):.3e}")
         result["split_bf16"]["tolerance"] = 2e-5

@@ -15,6 +15,7 @@
 #include "../../include/fastfft/fastfft.h"
 #include "launch.hpp"
 #include "plan.hpp"
+#include "tuning.hpp"
 
 using namespace fastfft_amd;
 
@@ -31,6 +32,31 @@ fastfft_status ok( ) { return FASTFFT_SUCCESS; }
 
 fastfft_status hip_fail(hipError_t e, const char* what) {
     return fail(FASTFFT_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device allocation of a plan's internal buffers (workspace, argmax scratch,
+// batch slots and frame).  With FASTFFT_POISON=1 (a debugging switch the GPU
+// tests turn on) every new buffer is filled with 0xff bytes, a NaN in every
+// float and float2 slot and -1 in every int, so that a kernel that reads
+// workspace nobody wrote puts NaN into its outputs.  Never called while a
+// stream is capturing (allocation is not allowed there either).
+hipError_t dev_alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if ( e != hipSuccess ) {
+        *p = nullptr;
+        return e;
+    }
+    const char* v = std::getenv("FASTFFT_POISON");
+    if ( v && std::atoi(v) != 0 ) {
+        e = hipMemset(*p, 0xff, bytes);
+        if ( e == hipSuccess )
+            e = hipDeviceSynchronize( );
+        if ( e != hipSuccess ) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    }
+    return e;
 }
 
 constexpr size_t kMaxLdsBytes = 160 * 1024; // gfx950 LDS per CU
@@ -103,6 +129,9 @@ struct fastfft_plan_s {
     // templates share one launch per stage; batch_dev holds the A and B slots of
     // a chunk, the argmax keys and the per-template launch descriptors
     int           batch_chunk = 0; // 0: per-template launches
+    int           batch_group = 0; // templates per template group of the correlation pass (0: the launcher's default)
+    int           batch_p     = 0; // sub-transform length of the correlation pass (0: the planner's)
+    std::string   batch_tuned;     // where the launch shape came from: env, cache or timed
     void*         batch_dev   = nullptr;
     RowBatchItem* batch_r2c   = nullptr;
     XcorrItem*    batch_xc    = nullptr;
@@ -122,6 +151,13 @@ struct fastfft_plan_s {
     void*              sub_dev = nullptr; // frame (real, FFTW padded), its spectrum, workspace a, b
     float*             sub_s   = nullptr;
     float2*            sub_h = nullptr, *sub_wa = nullptr, *sub_wb = nullptr;
+    // The library's own HIP graph of one batched call: captured on graph_stream at the
+    // first call with a given image and scales, then launched as one graph per call on
+    // the plan's stream (when that stream is not itself being captured).
+    hipStream_t    graph_stream = nullptr;
+    hipGraph_t     graph        = nullptr;
+    hipGraphExec_t graph_exec   = nullptr;
+    std::string    graph_key;
 };
 
 static int op_index(const char* op) {
@@ -146,7 +182,18 @@ static void clear_stats(fastfft_plan p) {
     p->stats_on    = false;
 }
 
+static void free_batch_graph(fastfft_plan p) {
+    if ( p->graph_exec )
+        (void)hipGraphExecDestroy(p->graph_exec);
+    if ( p->graph )
+        (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph      = nullptr;
+    p->graph_key.clear( );
+}
+
 static void free_batch_device(fastfft_plan p) {
+    free_batch_graph(p);
     clear_stats(p); // the statistics belong to the batch's kernels
     if ( p->batch_dev )
         (void)hipFree(p->batch_dev);
@@ -155,6 +202,9 @@ static void free_batch_device(fastfft_plan p) {
     p->batch_xc    = nullptr;
     p->batch_c2r   = nullptr;
     p->batch_chunk = 0;
+    p->batch_group = 0;
+    p->batch_p     = 0;
+    p->batch_tuned.clear( );
 }
 
 static void free_subframe(fastfft_plan p) {
@@ -222,7 +272,7 @@ static fastfft_status ensure_device_state(fastfft_plan p, float2** tw) {
         size_t bytes = sizeof(float2) * (p->ws_a + p->ws_b);
         if ( bytes < 256 )
             bytes = 256;
-        e = hipMalloc(&p->ws, bytes);
+        e = dev_alloc(&p->ws, bytes);
         if ( e != hipSuccess ) {
             p->ws = nullptr;
             return e == hipErrorOutOfMemory ? fail(FASTFFT_OUT_OF_MEMORY, "workspace allocation failed")
@@ -241,6 +291,11 @@ static fastfft_status op_kind_check(const fastfft_op& op, bool allow_scale, bool
     if ( (op.kind == FASTFFT_OP_CONJ_MUL || op.kind == FASTFFT_OP_CONJ_MUL_THEN_SCALE) && allow_conj )
         return ok( );
     return fail(FASTFFT_INVALID_ARGUMENT, std::string("operator kind ") + std::to_string(op.kind) + " is not valid as " + which);
+}
+
+static bool env_flag(const char* name) {
+    const char* v = std::getenv(name);
+    return v && v[0] && std::atoi(v) != 0;
 }
 
 static int env_chunk( ) {
@@ -271,7 +326,7 @@ static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stag
 static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
                                      void* out, const void* image, float pre, float intra, float post, int first = 0,
                                      int last = -1, const ArgmaxSlots* amax = nullptr, bool* amax_fused = nullptr,
-                                     float2* ws_a = nullptr, float2* ws_b = nullptr);
+                                     float2* ws_a = nullptr, float2* ws_b = nullptr, const float2* tw_in = nullptr);
 
 static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, const void* image, float pre, float intra,
                                  float post, int first = 0, int last = -1, const ArgmaxSlots* amax = nullptr,
@@ -283,18 +338,23 @@ static fastfft_status run_stages(fastfft_plan p, int opi, void* in, void* out, c
 }
 
 // ws_a / ws_b (both or neither): workspace of an internal stage list instead of the plan's.
+// tw_in with ws_a: the twiddle table too, so that nothing of the plan's device state is
+// allocated or used (the allocation-free entry point).
 static fastfft_status run_stage_list(fastfft_plan p, const std::vector<Stage>& stages, const char* opname, void* in,
                                      void* out, const void* image, float pre, float intra, float post, int first, int last,
-                                     const ArgmaxSlots* amax, bool* amax_fused, float2* ws_a, float2* ws_b) {
+                                     const ArgmaxSlots* amax, bool* amax_fused, float2* ws_a, float2* ws_b,
+                                     const float2* tw_in) {
     if ( last < 0 )
         last = (int)stages.size( ) - 1;
     if ( first < 0 || last >= (int)stages.size( ) || first > last )
         return fail(FASTFFT_INVALID_ARGUMENT,
                     "stage range outside the " + std::to_string(stages.size( )) + " stages of " + opname);
-    float2*        tw = nullptr;
-    fastfft_status rc = ensure_device_state(p, &tw);
-    if ( rc )
-        return rc;
+    float2* tw = const_cast<float2*>(tw_in);
+    if ( ! (tw_in && ws_a) ) {
+        fastfft_status rc = ensure_device_state(p, &tw);
+        if ( rc )
+            return rc;
+    }
     float2* wa = ws_a ? ws_a : static_cast<float2*>(p->ws);
     float2* wb = ws_a ? ws_b : wa + p->ws_a;
     auto    ptr = [&](Buf b) -> void* {
@@ -368,7 +428,7 @@ static fastfft_status argmax_scratch(fastfft_plan plan, ArgmaxScratch& a) {
     const size_t keys  = sizeof(unsigned long long) * kArgmaxPartials;
     const size_t bytes = keys + sizeof(float) * (kArgmaxPartials + 1) + sizeof(int) * (kArgmaxPartials + 2);
     if ( ! plan->argmax_scratch ) {
-        hipError_t e = hipMalloc(&plan->argmax_scratch, bytes);
+        hipError_t e = dev_alloc(&plan->argmax_scratch, bytes);
         if ( e != hipSuccess ) {
             plan->argmax_scratch = nullptr;
             return hip_fail(e, "hipMalloc(argmax scratch)");
@@ -384,9 +444,6 @@ static fastfft_status argmax_scratch(fastfft_plan plan, ArgmaxScratch& a) {
 }
 
 // ---------------------------------------------------------------- batched correlation
-// Chunk sizes chosen by timing, per batch shape, for the life of the process.
-static std::mutex                 g_chunk_mutex;
-static std::map<std::string, int> g_chunk_cache;
 
 // Elements a line map addresses for nlines lines of nelem elements (single-index lines).
 static size_t map_extent(const HostLineMap& m, int nlines, int nelem) {
@@ -443,10 +500,21 @@ enum : unsigned {
     kBatchAll     = 31u
 };
 
+// The correlation stage with sub-transform length P (0 or P = s.P: s itself).
+static Stage xcorr_stage_with_p(const Stage& s, int P) {
+    Stage x = s;
+    if ( P > 0 && P != s.P ) {
+        x.P = P;
+        x.Q = s.N / P;
+    }
+    return x;
+}
+
 // Launches every chunk of the batch: R2C rows, fused columns, C2R rows (+ argmax keys);
 // parts outside `parts` are skipped.
 static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>& st, const void* image, CpxT img_ct,
-                                       float pre, float intra, float post, const float2* tw, unsigned parts = kBatchAll) {
+                                       float pre, float intra, float post, const float2* tw, unsigned parts = kBatchAll,
+                                       bool mixed = false) {
     const int   n    = (int)p->batch.size( );
     const int   C    = p->batch_chunk;
     const bool  half = p->st.input_type == FASTFFT_INPUT_F16;
@@ -464,8 +532,10 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
             io.scale   = intra;
             io.xitems     = p->batch_xc + c0;
             io.nxitems    = cnt;
+            io.xgroup     = p->batch_group;
             io.split_bf16 = p->precision == FASTFFT_PRECISION_SPLIT_BF16;
-            e          = launch_xcorr(st[1], io, tw, p->stream);
+            io.img_mixed  = mixed;
+            e             = launch_xcorr(xcorr_stage_with_p(st[1], p->batch_p), io, tw, p->stream);
         }
         if ( e == hipSuccess && (parts & kBatchC2R) )
             e = p->stats_on ? launch_c2r_stats_batch(st[2], p->batch_c2r + c0, cnt, post, tw, p->stats_first_id + c0,
@@ -480,116 +550,193 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
     return ok( );
 }
 
-// Picks the chunk size: FASTFFT_BATCH_CHUNK if set, else the largest of 1, 2, 4,
-// ... templates per launch whose time on this batch (a zero image) is within 5 %
-// of the fastest, cached per batch shape.  Allocates batch_dev for the chosen size and fills the descriptors.
+static int next_pow2(int v);
+
+// The split-BF16 form of a batch on a reduced frame: the frame's forward list stops after
+// its R2C x-pass, whose output (the frame in mixed form, with the frame spectrum's column
+// layout) the matrix-core correlation pass reads in place of the spectrum.
+static bool batch_mixed(fastfft_plan p) {
+    if ( ! p->sub_active || p->precision != FASTFFT_PRECISION_SPLIT_BF16 || p->sub_fwd.size( ) != 2 ||
+         p->sub_xc.size( ) != 3 )
+        return false;
+    const Stage &f = p->sub_fwd[0], &x = p->sub_xc[1];
+    return f.kind == StageKind::fwd_real && (f.dst == Buf::ws_a || f.dst == Buf::ws_b) && f.om.s1 == 1 &&
+           f.om.n2 == 1 && f.om.es == x.gm.s1 && f.N == x.N && f.nout == x.nlines && xcorr_toeplitz_covers(x);
+}
+
+// Tuning-cache key of a batch: device architecture, precision, element types, the plan's
+// transform size, the frame the templates are correlated on, the largest template, the
+// window, the batch size (rounded up to a power of two, at most 128) and whether peaks
+// are computed.
+static std::string batch_tuning_key(fastfft_plan p, const std::vector<Stage>& st, CpxT img_ct) {
+    int lx = 0, ly = 0;
+    for ( const auto& it : p->batch ) {
+        lx = std::max(lx, it.x);
+        ly = std::max(ly, it.y);
+    }
+    const int   n  = std::min(next_pow2((int)p->batch.size( )), 128);
+    const auto& nn = p->st.inv_in;
+    const auto& mo = p->st.inv_out;
+    char        buf[320];
+    std::snprintf(buf, sizeof(buf), "%s %s %s/%s plan=%dx%d frame=%s L=%dx%d M=%dx%d o=%d,%d n=%d peaks=%d",
+                  device_arch( ).c_str( ), p->precision == FASTFFT_PRECISION_SPLIT_BF16 ? "split_bf16" : "fp32",
+                  p->st.input_type == FASTFFT_INPUT_F16 ? "f16" : "f32", img_ct == CpxT::c32h ? "c32h" : "c64", nn.x, nn.y,
+                  p->sub_active ? (std::to_string(p->sub_fx) + "x" + std::to_string(p->sub_fy)).c_str( ) : "full", lx, ly,
+                  mo.x, mo.y, p->sub_active ? 0 : p->st.origin[0], p->sub_active ? 0 : p->st.origin[1], n,
+                  p->batch_peaks != nullptr ? 1 : 0);
+    (void)st;
+    return buf;
+}
+
+// Chooses the batch's launch shape -- templates per launch (chunk), templates per group
+// of the correlation pass, and the pass's sub-transform length -- then allocates
+// batch_dev for it and fills the descriptors.  In order: FASTFFT_BATCH_CHUNK (with the
+// default group and length), the tuning cache (unless FASTFFT_TUNE=1), else timing every
+// candidate on this batch: the whole batched call, one warm-up and the best of three
+// runs, against a constant image, with the outputs and peaks redirected to scratch of
+// the plan's (the caller's buffers are not written); the fastest shape wins, a larger
+// chunk within 3 % of it preferred (fewer, larger launches replay better from a graph).
+// The timed choice is recorded in the tuning cache.
 static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stage>& st, CpxT img_ct) {
-    const int n = (int)p->batch.size( );
+    const int n  = (int)p->batch.size( );
     float2*   tw = nullptr;
     if ( auto rc = ensure_device_state(p, &tw) )
         return rc;
-    std::vector<std::pair<int, int>> sizes;
-    for ( const auto& it : p->batch )
-        sizes.push_back({it.x, it.y});
-    std::sort(sizes.begin( ), sizes.end( ));
-    std::string key = std::to_string(st[1].N) + "/" + std::to_string(st[1].P) + "/" + std::to_string(st[0].N) + "/" +
-                      std::to_string(st[2].M) + "x" + std::to_string(st[2].nlines) + "/" + std::to_string(st[1].o) + "/" +
-                      std::to_string(p->st.input_type) + std::to_string((int)img_ct) + "/" + std::to_string(p->batch_peaks != nullptr) +
-                      "/" + std::to_string(p->precision);
-    for ( const auto& s : sizes )
-        key += ":" + std::to_string(s.first) + "x" + std::to_string(s.second);
-    int chosen = env_chunk( );
-    if ( chosen <= 0 ) {
-        std::lock_guard<std::mutex> lock(g_chunk_mutex);
-        auto                        f = g_chunk_cache.find(key);
-        if ( f != g_chunk_cache.end( ) )
-            chosen = f->second;
+    free_batch_device(p);
+    const std::string key = batch_tuning_key(p, st, img_ct);
+    TuneEntry         best;
+    std::string       how;
+    if ( env_chunk( ) > 0 ) {
+        best.chunk = env_chunk( );
+        how        = "env";
     }
-    if ( chosen <= 0 ) {
-        // candidates up to the batch size and about 1 GiB of slots
-        const size_t per = sizeof(float2) * (slot_elems_a(st[0]) + slot_elems_b(st[2]));
-        std::vector<int> cand;
+    else if ( ! env_flag("FASTFFT_TUNE") && tuning_lookup(key, best) && best.chunk > 0 ) {
+        how = "cache";
+        // an entry from another build may name a length this batch's kernels do not take
+        if ( best.p > 0 && ! xcorr_col_covers(xcorr_stage_with_p(st[1], best.p), img_ct) )
+            best.p = 0;
+    }
+    else {
+        // candidate shapes (the matrix-core pass has one template group per 32 templates)
+        const bool       mixed = batch_mixed(p);
+        std::vector<int> chunks, groups = {1, 2, 4}, ps = {0};
+        if ( mixed )
+            groups = {0};
+        const size_t     per = sizeof(float2) * (slot_elems_a(st[0]) + slot_elems_b(st[2]));
         for ( int C = 1; C <= std::min(n, 64); C *= 2 )
-            if ( C == 1 || per * (size_t)C <= (size_t(1) << 30) )
-                cand.push_back(C);
-        const int cmax = cand.back( );
-        free_batch_device(p);
-        hipError_t e = hipMalloc(&p->batch_dev, batch_dev_bytes(st, cmax, n));
-        void*      img = nullptr;
+            if ( (C >= 8 || C == std::min(n, 64) || n < 8) && (C == 1 || per * (size_t)C <= (size_t(1) << 30)) )
+                chunks.push_back(C);
+        if ( chunks.empty( ) )
+            chunks.push_back(1);
+        // the 1024-point correlation pass as one wave per column (P = N) besides the planner's split
+        if ( p->precision == FASTFFT_PRECISION_FP32 && st[1].N == 1024 && st[1].P != 1024 &&
+             xcorr_col_covers(xcorr_stage_with_p(st[1], 1024), img_ct) )
+            ps.push_back(1024);
+        const int cmax = chunks.back( );
+        // scratch: the slots, descriptors and keys for the largest chunk, then one output
+        // per slot and the peaks and values of every template
+        const size_t out_bytes = elem_bytes(p->st) * (size_t)p->st.inv_output_memory;
+        const size_t bd        = (batch_dev_bytes(st, cmax, n) + 255) / 256 * 256;
         const size_t img_bytes = (img_ct == CpxT::c32h ? 4 : 8) * (size_t)st[1].gm.s1 * st[1].nlines;
+        const size_t outs_off  = bd;
+        const size_t pk_off    = outs_off + (out_bytes * cmax + 255) / 256 * 256;
+        const size_t img_off   = pk_off + ((sizeof(int) * 2 + sizeof(float)) * (size_t)n + 255) / 256 * 256;
+        void*        scratch   = nullptr;
+        hipError_t   e         = dev_alloc(&scratch, img_off + img_bytes);
+        if ( e != hipSuccess )
+            return hip_fail(e, "batch tuning scratch");
+        char* base = static_cast<char*>(scratch);
+        // a constant image (about 0.0115 in every float: no zeros, no denormals)
+        e = hipMemset(base + img_off, 0x3c, img_bytes);
         if ( e == hipSuccess )
-            e = hipMalloc(&img, img_bytes);
-        if ( e == hipSuccess )
-            e = hipMemset(img, 0, img_bytes);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+            e = hipDeviceSynchronize( );
+        // redirect the outputs and peaks for the timing runs
+        std::vector<void*> user_out(n);
+        int*               user_pk  = p->batch_peaks;
+        float*             user_val = p->batch_values;
+        for ( int i = 0; i < n; i++ ) {
+            user_out[i]      = p->batch[i].out;
+            p->batch[i].out = base + outs_off + out_bytes * (size_t)(i % cmax);
+        }
+        if ( user_pk ) {
+            p->batch_peaks  = reinterpret_cast<int*>(base + pk_off);
+            p->batch_values = reinterpret_cast<float*>(base + pk_off + sizeof(int) * 2 * (size_t)n);
+        }
+        p->batch_dev      = scratch;
+        hipEvent_t     e0 = nullptr, e1 = nullptr;
+        fastfft_status run = FASTFFT_SUCCESS; // a failed launch keeps its own message
         if ( e == hipSuccess )
             e = hipEventCreate(&e0);
         if ( e == hipSuccess )
             e = hipEventCreate(&e1);
-        float              best = 0.f;
-        std::vector<float> times;
-        fastfft_status     run = FASTFFT_SUCCESS; // a failed launch keeps its own message
-        chosen                 = 1;
-        for ( size_t ci = 0; e == hipSuccess && ci < cand.size( ); ci++ ) {
-            p->batch_chunk = cand[ci];
-            if ( (run = fill_batch_items(p, st, cand[ci])) != FASTFFT_SUCCESS )
-                break;
-            // one warm-up run, then the fastest of kTuneReps timed runs (robust to a
-            // neighbour's interference on a shared device)
-            constexpr int kTuneReps = 5;
-            float         ms        = 0.f;
-            for ( int rep = 0; rep <= kTuneReps && e == hipSuccess && ! run; rep++ ) {
-                e = hipEventRecord(e0, p->stream);
-                if ( e == hipSuccess )
-                    run = run_batch_chunks(p, st, img, img_ct, 1.f, 1.f, 1.f, tw);
-                if ( run )
-                    break;
-                if ( e == hipSuccess )
-                    e = hipEventRecord(e1, p->stream);
-                if ( e == hipSuccess )
-                    e = hipEventSynchronize(e1);
-                float t = 0.f;
-                if ( e == hipSuccess )
-                    e = hipEventElapsedTime(&t, e0, e1);
-                if ( rep == 1 || (rep > 1 && t < ms) )
-                    ms = t;
-            }
-            if ( run )
-                break;
-            if ( e == hipSuccess )
-                times.push_back(ms);
-        }
-        // the largest chunk within 5 % of the fastest: fewer, larger launches replay
-        // better from a graph than eager timing shows (at 64 x 384^2 -> 512^2, chunk 64
-        // replays in 286 us against 292-304 us for chunk 32, which times faster eagerly)
-        if ( ! run && e == hipSuccess && times.size( ) == cand.size( ) ) {
-            best = *std::min_element(times.begin( ), times.end( ));
-            for ( size_t ci = 0; ci < cand.size( ); ci++ )
-                if ( times[ci] <= 1.05f * best )
-                    chosen = cand[ci];
-        }
+        best = TuneEntry{ };
+        std::vector<TuneEntry> timed;
+        for ( int pc : ps )
+            for ( int g : groups )
+                for ( int C : chunks ) {
+                    if ( e != hipSuccess || run || g > C )
+                        continue; // (g = 0: the kernel's own grouping)
+                    p->batch_chunk = C;
+                    p->batch_group = g;
+                    p->batch_p     = pc;
+                    if ( (run = fill_batch_items(p, st, C)) != FASTFFT_SUCCESS )
+                        continue;
+                    float ms = 0.f;
+                    for ( int rep = 0; rep < 4 && e == hipSuccess && ! run; rep++ ) {
+                        e = hipEventRecord(e0, p->stream);
+                        if ( e == hipSuccess )
+                            run = run_batch_chunks(p, st, base + img_off, img_ct, 1.f, 1.f, 1.f, tw, kBatchAll, mixed);
+                        if ( e == hipSuccess && ! run )
+                            e = hipEventRecord(e1, p->stream);
+                        if ( e == hipSuccess && ! run )
+                            e = hipEventSynchronize(e1);
+                        float t = 0.f;
+                        if ( e == hipSuccess && ! run )
+                            e = hipEventElapsedTime(&t, e0, e1);
+                        if ( rep == 1 || (rep > 1 && t < ms) )
+                            ms = t;
+                    }
+                    if ( e != hipSuccess || run )
+                        continue;
+                    timed.push_back(TuneEntry{C, g, pc, (double)ms});
+                }
+        // the fastest, or a larger chunk within 3 % of it (the fastest of those)
+        double tmin = 0.0;
+        for ( const TuneEntry& c : timed )
+            if ( tmin == 0.0 || c.ms < tmin )
+                tmin = c.ms;
+        for ( const TuneEntry& c : timed )
+            if ( c.ms <= 1.03 * tmin &&
+                 (best.chunk == 0 || c.chunk > best.chunk || (c.chunk == best.chunk && c.ms < best.ms)) )
+                best = c;
         if ( e0 )
             (void)hipEventDestroy(e0);
         if ( e1 )
             (void)hipEventDestroy(e1);
-        if ( img )
-            (void)hipFree(img);
-        free_batch_device(p);
+        for ( int i = 0; i < n; i++ )
+            p->batch[i].out = user_out[i];
+        p->batch_peaks  = user_pk;
+        p->batch_values = user_val;
+        free_batch_device(p); // frees the scratch
         if ( run )
             return run;
         if ( e != hipSuccess )
-            return hip_fail(e, "batch chunk tuning");
-        std::lock_guard<std::mutex> lock(g_chunk_mutex);
-        g_chunk_cache[key] = chosen;
+            return hip_fail(e, "batch tuning");
+        if ( best.chunk <= 0 )
+            return fail(FASTFFT_HIP_ERROR, "batch tuning: no candidate shape ran");
+        tuning_store(key, best);
+        how = "timed";
     }
-    free_batch_device(p);
-    hipError_t e = hipMalloc(&p->batch_dev, batch_dev_bytes(st, chosen, n));
+    hipError_t e = dev_alloc(&p->batch_dev, batch_dev_bytes(st, best.chunk, n));
     if ( e != hipSuccess ) {
         p->batch_dev = nullptr;
         return hip_fail(e, "hipMalloc(batch)");
     }
-    p->batch_chunk = chosen;
-    return fill_batch_items(p, st, chosen);
+    p->batch_chunk = best.chunk;
+    p->batch_group = best.group;
+    p->batch_p     = best.p;
+    p->batch_tuned = how;
+    return fill_batch_items(p, st, best.chunk);
 }
 
 static int next_pow2(int v) {
@@ -688,7 +835,7 @@ static fastfft_status setup_subframe(fastfft_plan p) {
     const size_t wa = std::max(ia, fa), wb = std::max(ib, fb);
     s_floats        = (s_floats + 63) / 64 * 64;
     const size_t bytes = sizeof(float) * s_floats + sizeof(float2) * (h_elems + wa + wb) + 256;
-    hipError_t   e     = hipMalloc(&p->sub_dev, bytes);
+    hipError_t   e     = dev_alloc(&p->sub_dev, bytes);
     if ( e != hipSuccess ) {
         p->sub_dev = nullptr;
         return hip_fail(e, "hipMalloc(batch frame)");
@@ -710,6 +857,7 @@ static fastfft_status setup_subframe(fastfft_plan p) {
 static fastfft_status run_subframe_batch(fastfft_plan p, const void* image, float pre, float intra, float post,
                                          const float2* tw, unsigned parts = kBatchAll) {
     const float frame_scale = 1.f / ((float)p->sub_fx * (float)p->sub_fy);
+    const bool  mixed       = batch_mixed(p);
     if ( parts & kBatchFrame ) {
         const auto &inv = p->sub_inv, &fwd = p->sub_fwd;
         // the inverse's C2R rows and the forward's R2C rows are the frame's rows: one kernel
@@ -735,11 +883,64 @@ static fastfft_status run_subframe_batch(fastfft_plan p, const void* image, floa
             if ( e != hipSuccess )
                 return hip_fail(e, "batch frame rows");
         }
-        if ( auto rc = run_stage_list(p, fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f,
-                                      fuse ? 1 : 0, -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
-            return rc;
+        // the frame's spectrum; in the split-BF16 form only its R2C x-pass (already done by
+        // the fused kernel)
+        if ( ! (mixed && fuse) )
+            if ( auto rc = run_stage_list(p, fwd, "batch frame spectrum", p->sub_s, p->sub_h, nullptr, 1.f, 1.f, 1.f,
+                                          fuse ? 1 : 0, mixed ? 0 : -1, nullptr, nullptr, p->sub_wa, p->sub_wb) )
+                return rc;
     }
-    return run_batch_chunks(p, p->sub_xc, p->sub_h, CpxT::c64, pre, intra, post, tw, parts);
+    const void* img = mixed ? (p->sub_fwd[0].dst == Buf::ws_a ? p->sub_wa : p->sub_wb) : static_cast<const void*>(p->sub_h);
+    return run_batch_chunks(p, p->sub_xc, img, CpxT::c64, pre, intra, post, tw, parts, mixed);
+}
+
+// Runs one batched call through the plan's own HIP graph: captured at the first call
+// with this image and these scales (on a stream of the plan's, in relaxed capture mode:
+// the call issues kernel launches only), then one graph launch on the plan's stream per
+// call.  Calls on a stream that is itself being captured, calls in statistics form (whose
+// template ids change from call to call) and FASTFFT_NO_BATCH_GRAPH=1 launch the kernels
+// directly.
+template <class Run>
+static fastfft_status run_batch_graph(fastfft_plan p, const void* image, float pre, float intra, float post, Run&& run) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if ( p->stats_on || env_flag("FASTFFT_NO_BATCH_GRAPH") || hipStreamIsCapturing(p->stream, &cs) != hipSuccess ||
+         cs != hipStreamCaptureStatusNone )
+        return run( );
+    char key[160];
+    std::snprintf(key, sizeof(key), "%p/%a/%a/%a", image, (double)pre, (double)intra, (double)post);
+    if ( ! p->graph_exec || p->graph_key != key ) {
+        free_batch_graph(p);
+        hipError_t e = hipSuccess;
+        if ( ! p->graph_stream )
+            e = hipStreamCreateWithFlags(&p->graph_stream, hipStreamNonBlocking);
+        if ( e != hipSuccess ) {
+            p->graph_stream = nullptr;
+            return hip_fail(e, "batch graph stream");
+        }
+        const hipStream_t user = p->stream;
+        p->stream              = p->graph_stream;
+        e                      = hipStreamBeginCapture(p->graph_stream, hipStreamCaptureModeRelaxed);
+        fastfft_status rc      = e == hipSuccess ? run( ) : FASTFFT_SUCCESS;
+        hipGraph_t     g       = nullptr;
+        if ( e == hipSuccess ) {
+            const hipError_t ee = hipStreamEndCapture(p->graph_stream, &g);
+            if ( ! rc && ee != hipSuccess )
+                e = ee;
+        }
+        p->stream = user;
+        if ( e == hipSuccess && ! rc )
+            e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+        if ( rc || e != hipSuccess ) {
+            if ( g )
+                (void)hipGraphDestroy(g);
+            p->graph_exec = nullptr;
+            return rc ? rc : hip_fail(e, "batch graph capture");
+        }
+        p->graph     = g;
+        p->graph_key = key;
+    }
+    const hipError_t e = hipGraphLaunch(p->graph_exec, p->stream);
+    return e == hipSuccess ? ok( ) : hip_fail(e, "batch graph launch");
 }
 
 extern "C" {
@@ -773,6 +974,8 @@ fastfft_status fastfft_destroy(fastfft_plan plan) {
         (void)hipFree(plan->argmax_scratch);
     free_batch_device(plan);
     free_subframe(plan);
+    if ( plan->graph_stream )
+        (void)hipStreamDestroy(plan->graph_stream);
     delete plan;
     return ok( );
 }
@@ -1038,11 +1241,13 @@ fastfft_status fastfft_fwd_image_inv_batch(fastfft_plan plan, const void* image_
         float2* tw = nullptr;
         if ( auto rc = ensure_device_state(plan, &tw) )
             return rc;
-        const CpxT img_ct = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
-        if ( auto rc = plan->sub_active ? run_subframe_batch(plan, image_to_search, op_scale(pre_op), op_scale(intra_op),
-                                                             op_scale(post_op), tw)
-                                        : run_batch_chunks(plan, *sp, image_to_search, img_ct, op_scale(pre_op),
-                                                           op_scale(intra_op), op_scale(post_op), tw) )
+        const CpxT  img_ct = plan->st.image_type == FASTFFT_IMAGE_C32H ? CpxT::c32h : CpxT::c64;
+        const float pre = op_scale(pre_op), intra = op_scale(intra_op), post = op_scale(post_op);
+        auto        run = [&]( ) {
+            return plan->sub_active ? run_subframe_batch(plan, image_to_search, pre, intra, post, tw)
+                                           : run_batch_chunks(plan, *sp, image_to_search, img_ct, pre, intra, post, tw);
+        };
+        if ( auto rc = run_batch_graph(plan, image_to_search, pre, intra, post, run) )
             return rc;
         if ( ! plan->batch.empty( ) ) {
             plan->last_buffer     = plan->batch.back( ).out;
@@ -1268,6 +1473,7 @@ fastfft_status fastfft_set_batch_stats(fastfft_plan plan, float* mip, int* best,
         return rc;
     if ( ! mip && ! best && ! sum && ! sum_sq ) {
         clear_stats(plan);
+        free_batch_graph(plan);
         return ok( );
     }
     if ( ! mip || ! best || ! sum || ! sum_sq )
@@ -1282,7 +1488,7 @@ fastfft_status fastfft_set_batch_stats(fastfft_plan plan, float* mip, int* best,
     float* part = plan->batch_stats.part;
     if ( ! part ) {
         const size_t bytes = sizeof(float) * 4 * (size_t)kStatsGroups * (size_t)c2r->M * (size_t)c2r->nlines;
-        if ( hipMalloc(&part, bytes) != hipSuccess ) {
+        if ( dev_alloc(reinterpret_cast<void**>(&part), bytes) != hipSuccess ) {
             (void)hipGetLastError( );
             part = nullptr; // one group per launch
         }
@@ -1290,6 +1496,7 @@ fastfft_status fastfft_set_batch_stats(fastfft_plan plan, float* mip, int* best,
     plan->batch_stats    = BatchStats{mip, best, sum, sum_sq, part};
     plan->stats_first_id = first_id;
     plan->stats_on       = true;
+    free_batch_graph(plan);
     return ok( );
 }
 
@@ -1316,7 +1523,11 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
     if ( std::strcmp(op, "batch") == 0 ) {
         // the batch set by fastfft_set_batch: its form, chunk size and internal stage lists
         s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk) +
-            " precision=" + (plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? "split_bf16" : "fp32");
+            " group=" + std::to_string(plan->batch_group) + " p=" + std::to_string(plan->batch_p) +
+            " tuned=" + (plan->batch_tuned.empty( ) ? std::string("none") : plan->batch_tuned) +
+            " precision=" + (batch_mixed(plan) ? std::string("split_bf16")
+                             : plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? std::string("fp32 (split_bf16 does not apply)")
+                                                                               : std::string("fp32"));
         if ( plan->sub_active ) {
             s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) + "\n";
             s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +

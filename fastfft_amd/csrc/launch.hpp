@@ -37,9 +37,15 @@ struct StageIO {
     // array of XcorrItem, xcorr_col.hpp) instead of src / dst
     const void* xitems  = nullptr;
     int         nxitems = 0;
+    // xcorr with xitems: templates per template group along gridDim.y (0: the default)
+    int xgroup = 0;
     // xcorr only: run the split-BF16 matrix-core kernel where it covers the stage
-    // (xcorr_mfma_covers), the FP32 kernels elsewhere
+    // (xcorr_toeplitz_covers, with img_mixed), the FP32 kernels elsewhere
     bool split_bf16 = false;
+    // xcorr only: img is the frame in mixed form, the frame's R2C x-pass output R2C_x(s)
+    // (column kx of N positions at img + kx * s.gm.s1) instead of its spectrum; only the
+    // split-BF16 Toeplitz kernel takes it (the correlation then carries a factor N)
+    bool img_mixed = false;
 };
 
 // Batched correlation over n templates with the stage parameters of `s` (the
@@ -54,9 +60,11 @@ bool       xcorr_col_covers(const Stage& s, CpxT img_ct);
 // hipErrorNotSupported, before launching anything, when it does not cover s
 bool       tile_lines_cover(const Stage& s);
 hipError_t launch_tile_lines(const Stage& s, const StageIO& io, const float2* tw, hipStream_t stream);
-// the split-BF16 matrix-core correlation pass (xcorr_mfma.hpp): N = 1024, L <= 512,
-// window origin 0, single-index lines with unit line stride, FP32 image
-bool       xcorr_mfma_covers(const Stage& s, CpxT img_ct);
+// the split-BF16 matrix-core correlation pass (xcorr_toeplitz.hpp) on a frame in mixed
+// form: N = 1024, L <= 512, M <= 512, L + M - 1 <= 1024, window origin 0, the DC and
+// Nyquist columns of a real transform at both ends, row-major template and output
+// columns (unit line stride), unit-stride image columns
+bool       xcorr_toeplitz_covers(const Stage& s);
 bool       batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r, CpxT img_ct);
 hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
                                  const float2* tw, hipStream_t stream);
@@ -115,5 +123,6 @@ hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, flo
 
 // LDS bytes per workgroup of a line stage with sub-transform size P.
 size_t stage_lds_bytes(int P);
+
 
 } // namespace fastfft_amd

@@ -145,12 +145,15 @@ static hipError_t try_c2r_packed(const Stage& s, const StageIO& io, const float2
 // Trimmed C2R passes of 2048, 4096 or 8192 points (and full-length ones asked
 // for their argmax): the packed windowed kernel, Q = N / 1024 waves per row
 // (rows_packed.hpp), with its argmax keys reduced after it.  Full-length passes
-// without argmax keep k_inv_full_wave.
+// without argmax keep k_inv_full_wave.  FASTFFT_NO_PACKED_WINDOW switches this
+// kernel off alone, FASTFFT_NO_WINDOW_ROWS together with the windowed row
+// kernel (the generic line kernel then takes these shapes).
 static hipError_t try_c2r_window_packed(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( s.kind != StageKind::inv_c2r || ! (s.N == 2048 || s.N == 4096 || s.N == 8192) || s.M < 1 || s.M > s.N ||
          (s.M == s.N && s.o % s.N == 0 && ! io.amax) ||
          s.nlines < 1 || io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real ||
-         env_int("FASTFFT_NO_PACKED_ROWS", 0) || env_int("FASTFFT_NO_PACKED_WINDOW", 0) )
+         env_int("FASTFFT_NO_PACKED_ROWS", 0) || env_int("FASTFFT_NO_PACKED_WINDOW", 0) ||
+         env_int("FASTFFT_NO_WINDOW_ROWS", 0) )
         return hipErrorNotSupported;
     const bool          keys = io.amax && s.nlines <= kArgmaxPartials;
     unsigned long long* rk   = keys ? io.amax->keys : nullptr;

@@ -10,7 +10,7 @@
 
 #include "launch_common.hpp"
 #include "xcorr_col.hpp"
-#include "xcorr_mfma.hpp"
+#include "xcorr_toeplitz.hpp"
 
 namespace fastfft_amd {
 
@@ -44,8 +44,9 @@ static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* 
     const int blocks = s.nlines - pack;
     int       groups = 1;
     if ( io.xitems && io.nxitems > 1 && blocks % 8 == 0 ) {
-        groups = env_int("FASTFFT_XCORR_GROUPS", (io.nxitems + kXcorrGroupTemplates - 1) / kXcorrGroupTemplates);
-        groups = std::max(1, std::min(groups, io.nxitems));
+        const int per = io.xgroup > 0 ? io.xgroup : kXcorrGroupTemplates;
+        groups        = env_int("FASTFFT_XCORR_GROUPS", (io.nxitems + per - 1) / per);
+        groups        = std::max(1, std::min(groups, io.nxitems));
     }
     XcorrItem one{io.src, io.dst, s.L, 0};
     k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks, (unsigned)groups), dim3(64 * Q), 0, st>>>(
@@ -62,7 +63,7 @@ static hipError_t try_xcorr_col(const Stage& s, const StageIO& io, const float2*
     const int rs = (((s.o / 64) % E) + E) % E;
     auto      by_q = [&](auto qc) -> hipError_t {
         constexpr int Q = decltype(qc)::value;
-        if constexpr ( P * Q > kTwiddleN ) {
+        if constexpr ( P * Q > kTwiddleN || (Q == 1 && P != 1024) ) {
             return hipErrorNotSupported;
         }
         else {
@@ -74,6 +75,7 @@ static hipError_t try_xcorr_col(const Stage& s, const StageIO& io, const float2*
         }
     };
     switch ( s.Q ) {
+        case 1: return by_q(std::integral_constant<int, 1>{ });
         case 2: return by_q(std::integral_constant<int, 2>{ });
         case 4: return by_q(std::integral_constant<int, 4>{ });
         case 8: return by_q(std::integral_constant<int, 8>{ });
@@ -89,33 +91,31 @@ bool xcorr_col_covers(const Stage& s, CpxT img_ct) {
     if ( ! (single_index_lines(s.im) && single_index_lines(s.om) && single_index_lines(s.gm) && s.gm.es == 1 &&
             s.gm.s1 % per == 0) )
         return false;
-    if ( ! (s.P == 256 || s.P == 512 || s.P == 1024) || s.o % 64 != 0 || s.P * s.Q > kTwiddleN ||
-         ! (s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) )
+    if ( ! (s.P == 256 || s.P == 512 || s.P == 1024) || s.o % 64 != 0 || s.P * s.Q > kTwiddleN || s.P < s.L ||
+         ! ((s.Q == 1 && s.P == 1024) || s.Q == 2 || s.Q == 4 || s.Q == 8 || s.Q == 16) )
         return false;
     const int E  = s.P / 64;
     const int rs = (((s.o / 64) % E) + E) % E;
     return rs == 0 || rs == E / 2;
 }
 
-bool xcorr_mfma_covers(const Stage& s, CpxT img_ct) {
-    return s.kind == StageKind::xcorr && s.N == 1024 && s.L >= 1 && s.L <= 512 && s.o == 0 && s.M >= 1 && s.M <= 1024 &&
-           img_ct == CpxT::c64 && single_index_lines(s.im) && s.im.s1 == 1 && single_index_lines(s.om) && s.om.s1 == 1 &&
-           single_index_lines(s.gm) && s.gm.es == 1;
+bool xcorr_toeplitz_covers(const Stage& s) {
+    return s.kind == StageKind::xcorr && s.N == kTzFy && s.L >= 1 && (s.L + 7) / 8 * 8 <= kTzMaxM && s.M >= 1 &&
+           s.M <= kTzMaxM && s.L + s.M - 1 <= s.N && s.o == 0 && s.real_end_lines && s.nlines >= 2 &&
+           single_index_lines(s.im) && s.im.s1 == 1 && single_index_lines(s.om) && s.om.s1 == 1 &&
+           single_index_lines(s.gm) && s.gm.es == 1 && ! env_int("FASTFFT_NO_TOEPLITZ", 0);
 }
 
-// Template groups per column of kXcorrGroupTemplates templates, as for the FP32
-// pass (at 64 x 384^2 on the 1024 frame: 326 us per 64-template launch against
-// 376 us with 16 per group); FASTFFT_MFMA_GROUPS overrides.
-static hipError_t go_xcorr_mfma(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    const int n      = io.xitems ? io.nxitems : 1;
-    int       groups = env_int("FASTFFT_MFMA_GROUPS", 0);
-    if ( groups <= 0 )
-        groups = (n + kXcorrGroupTemplates - 1) / kXcorrGroupTemplates;
-    groups = std::max(1, std::min(groups, n));
-    XcorrItem one{io.src, io.dst, s.L, 0};
-    k_xcorr_mfma1024<<<dim3((unsigned)s.nlines, (unsigned)groups), dim3(256), 0, st>>>(
-            one, static_cast<const XcorrItem*>(io.xitems), n, s.im.es, static_cast<const float2*>(io.img), s.gm.s1,
-            s.om.es, s.M, io.scale, tw);
+// One workgroup per column (the DC and Nyquist columns share workgroup 0) and template
+// group of 32; the frame's N-point y transforms are not taken, so the products carry N.
+static hipError_t go_xcorr_toeplitz(const Stage& s, const StageIO& io, hipStream_t st) {
+    const int n      = io.nxitems;
+    const int groups = (n + kTzTemplates - 1) / kTzTemplates;
+    if ( ! io.xitems || n < 1 )
+        return hipErrorNotSupported;
+    k_xcorr_toeplitz<<<dim3((unsigned)(s.nlines - 1), (unsigned)groups), dim3(kTzWaves * 64), 0, st>>>(
+            static_cast<const XcorrItem*>(io.xitems), n, s.im.es, static_cast<const float2*>(io.img), s.gm.s1, s.om.es, s.M,
+            s.L, io.scale * (float)s.N, s.nlines - 1, 1);
     return hipGetLastError( );
 }
 
@@ -126,8 +126,12 @@ bool batch_kernels_cover(const Stage& r2c, const Stage& xcorr, const Stage& c2r,
 }
 
 hipError_t launch_xcorr(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    if ( io.split_bf16 && xcorr_mfma_covers(s, io.img_ct) )
-        return go_xcorr_mfma(s, io, tw, st);
+    if ( io.img_mixed ) {
+        // a frame in mixed form is only taken by the matrix-core kernel
+        if ( ! (io.split_bf16 && io.img_ct == CpxT::c64 && xcorr_toeplitz_covers(s)) )
+            return hipErrorNotSupported;
+        return go_xcorr_toeplitz(s, io, st);
+    }
     // 16-byte image loads: aligned image base (the column starts are checked with the shape)
     const bool col_ok = xcorr_col_covers(s, io.img_ct) && reinterpret_cast<std::uintptr_t>(io.img) % 16 == 0 &&
                         ! env_int("FASTFFT_NO_WAVE_XCORR", 0);

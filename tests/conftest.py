@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# every workspace, batch slot and scratch buffer the library allocates starts as NaN bytes, so that a
+# kernel reading memory no kernel wrote shows up in the outputs the tests compare
+os.environ.setdefault("FASTFFT_POISON", "1")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU; run with -m gpu")

@@ -26,6 +26,14 @@ def dev_zeros(n_elems: int, dtype: str = "f32"):
     return torch.zeros(int(n_elems), dtype=td, device=d)
 
 
+def dev_poison(n_elems: int, dtype: str = "f32"):
+    """An output buffer filled with NaN: whatever a transform leaves unwritten stays NaN, and a window
+    that reads it fails the comparison."""
+    torch, d = torch_dev()
+    td = {"f32": torch.float32, "f16": torch.float16}[dtype]
+    return torch.full((int(n_elems),), float("nan"), dtype=td, device=d)
+
+
 def to_dev(a: np.ndarray):
     torch, d = torch_dev()
     return torch.from_numpy(np.ascontiguousarray(a)).to(d)

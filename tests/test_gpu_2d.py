@@ -73,7 +73,7 @@ def test_golden_inverse(golden):
         ft = H.make_plan((n_x, n_y, 1), (n_x, n_y, 1), (n_x, n_y, 1), (out_x, out_y, 1), origin=(ox, oy, 0))
         spec = golden[f"{key}_inv_in"]
         inp = H.to_dev(H.complex_to_f32(spec))
-        out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+        out = H.dev_poison(ft.ReturnInvOutputMemorySize())
         ft.InvFFT(inp, out)
         got = H.real_window(H.to_host(out), out_x, out_y)
         want = golden[f"{key}_inv_out"]
@@ -91,7 +91,7 @@ def test_golden_xcorr(golden):
     tin = golden["xc_tmpl"].reshape(-1)
     t[: tin.size] = H.to_dev(tin)
     img = H.to_dev(H.complex_to_f32(golden["xc_image_spec"]))
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.FwdImageInvFFT(t, img, out)
     got = H.real_window(H.to_host(out), out_x, out_y)
     want32 = O.xcorr_2d(golden["xc_tmpl"], tx, golden["xc_image_spec"].astype(np.complex64), n_x, n_y, out_x, out_y, ox, oy)
@@ -243,12 +243,15 @@ INV_CASES = [((512, 512), (512, 512), (0, 0)), ((4096, 4096), (512, 512), (-256,
              ((8192, 64), (3000, 64), (-1000, 0))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_window", "no_packed", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_window", "no_packed", "no_packed_window", "generic"])
 @pytest.mark.parametrize("nin,nout,origin", INV_CASES)
 def test_inv_random(nin, nout, origin, mode, monkeypatch):
     """Trimmed inverses against the FP64 oracle with the default kernel choice, without the windowed
-    inverse-row kernel (the generic line kernel takes its shapes), without the packed C2R rows, and
-    with the generic kernel only."""
+    inverse-row kernels (the packed 2048-8192-point C2R rows and the two-wave windowed rows: the generic
+    line kernel takes their shapes), without any packed C2R rows, without only the packed windowed C2R
+    rows, and with the generic kernel only."""
+    if mode == "no_packed_window":
+        monkeypatch.setenv("FASTFFT_NO_PACKED_WINDOW", "1")
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "no_packed":
@@ -261,7 +264,7 @@ def test_inv_random(nin, nout, origin, mode, monkeypatch):
     spec = (rng.standard_normal((w, nin[1])) + 1j * rng.standard_normal((w, nin[1]))).astype(np.complex64)
     ft = H.make_plan((*nin, 1), (*nin, 1), (*nin, 1), (*nout, 1), origin=(*origin, 0))
     inp = H.to_dev(H.complex_to_f32(spec))
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.InvFFT(inp, out)
     got = H.real_window(H.to_host(out), nout[0], nout[1])
     want = O.inv_2d(spec, nin[0], nin[1], nout[0], nout[1], *origin)
@@ -285,7 +288,7 @@ def test_xcorr_random(tin, n, nout, origin):
     tp = H.padded_real(t, tin[0]).reshape(-1)
     a[: tp.size] = H.to_dev(tp)
     img = H.to_dev(H.complex_to_f32(spec))
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.FwdImageInvFFT(a, img, out)
     got = H.real_window(H.to_host(out), nout[0], nout[1])
     want = O.xcorr_2d(H.padded_real(t, tin[0]), tin[0], spec, n[0], n[1], nout[0], nout[1], *origin)

@@ -56,7 +56,7 @@ def test_inv_3d_vs_oracle(n, nout, origin):
         spec = W.hermitian_spectrum_3d(n[0], seed=3)
     ft = H.make_plan(n, n, n, nout, origin=origin, rank=3)
     a = H.to_dev(H.complex_to_f32(spec))
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.InvFFT(a, out)
     got = H.real_window(H.to_host(out), *nout)
     want = O.inv_3d(spec, *n, *nout, *origin)
@@ -71,7 +71,7 @@ def test_round_trip_3d():
     buf = H.to_dev(H.padded_real(v, n[0]).reshape(-1))
     spec = H.dev_zeros(ft.ReturnFwdOutputMemorySize())
     ft.FwdFFT(buf, spec)
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.InvFFT(spec, out)
     got = H.real_window(H.to_host(out), *n)
     assert O.rel_l2(got, v.astype(np.float64) * np.prod(n)) < TOL
@@ -89,7 +89,7 @@ def test_xcorr_3d_vs_oracle(tin, n, nout, origin):
     tp = H.padded_real(t, tin[0]).reshape(-1)
     a[: tp.size] = H.to_dev(tp)
     g = H.to_dev(H.complex_to_f32(spec_img))
-    out = H.dev_zeros(ft.ReturnInvOutputMemorySize())
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize())
     ft.FwdImageInvFFT(a, g, out)
     got = H.real_window(H.to_host(out), *nout)
     # the oracle's correlation = InvFFT(image * conj(FwdFFT(template)))

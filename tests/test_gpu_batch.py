@@ -2,7 +2,9 @@
 
 Each output of a batch must agree with the single-template FwdImageInvFFT of the same template (the
 batched kernels use the sub-transform sizes of the largest template, so agreement is to rounding), each
-peak must equal the argmax of its output, and small batches are also checked against the FP64 oracle.  The batch is replayed from a HIP graph with eager work in between.
+peak must equal the argmax of its output, and the outputs and peaks of the small batches and of the batches
+on a reduced frame are also checked against the FP64 oracle of the whole N-frame pipeline.  The batch is
+replayed from a HIP graph with eager work in between.
 
 Where the window and the largest template fit a frame of at most a quarter of the image (Fx >= Lx + Mx - 1,
 Fy >= Ly + My - 1), the batch reduces the image to that frame once per call and correlates every template
@@ -38,7 +40,7 @@ def _setup(sizes, n, nout, origin, seed=0):
             rng.standard_normal((y, x)).astype(np.float32)
         hosts.append(t)
         tmpls.append(H.to_dev(H.padded_real(t, x).reshape(-1)))
-        outs.append(H.dev_zeros(ft.ReturnInvOutputMemorySize()))
+        outs.append(H.dev_poison(ft.ReturnInvOutputMemorySize()))
     peaks = torch.full((2 * len(sizes),), -1, dtype=torch.int32, device=d)
     vals = torch.zeros(len(sizes), dtype=torch.float32, device=d)
     ft.SetBatch(tmpls, sizes, outs, peaks, vals)
@@ -58,12 +60,15 @@ FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "
           ((2048, 2048), (512, 400)): "1024x1024", ((8192, 1024), (512, 64)): "1024x512"}
 
 
-@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "full_frame", "per_template"])
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "no_frame_fusion", "full_frame", "per_template"])
 @pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
 def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
     """Batched kernels (chunk size from the tuner, or 2 to force several chunks, or the two-wave row
-    kernels in place of the packed ones) and the per-template launches all agree with single-template
-    correlations and give the argmax of their own outputs."""
+    kernels in place of the packed ones, or the frame's C2R and R2C rows as two kernels) and the
+    per-template launches all agree with single-template correlations and give the argmax of their own
+    outputs."""
+    if mode == "no_frame_fusion":
+        monkeypatch.setenv("FASTFFT_NO_FRAME_FUSION", "1")
     if mode == "chunk2":
         monkeypatch.setenv("FASTFFT_BATCH_CHUNK", "2")
     if mode == "no_packed":
@@ -75,7 +80,7 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
     H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
     form = ft.Describe("batch").splitlines()[0]
-    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed") else None
+    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed", "no_frame_fusion") else None
     assert f"frame={frame or 'full'}" in form, form
     ft.FwdImageInvFFTBatch(img)
     pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
@@ -90,9 +95,32 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         assert O.rel_l2(got, H.real_window(H.to_host(o), *nout)) < TOL, i
         rx, ry, rv = single.ArgMax(outs[i], nout[0], nout[1], pitch)  # argmax of the batch's own output
         assert tuple(pk[i]) == (rx, ry) and vv[i] == np.float32(rv), i
-        if n[0] * n[1] <= 256 * 128:
+        # the FP64 oracle of the full N-frame pipeline: every small batch, and the batches on a reduced
+        # frame in the modes that run the frame (the fused frame rows at Q = 2, 4 and 8 included)
+        if n[0] * n[1] <= 256 * 128 or (frame and mode in ("tuned", "no_packed", "no_frame_fusion")):
             want = O.xcorr_2d(H.padded_real(hosts[i], x), x, spec, n[0], n[1], nout[0], nout[1], *origin)
             assert O.rel_l2(got, want) < TOL, i
+            wx, wy, _ = O.argmax_2d(want)
+            assert tuple(pk[i]) == (wx, wy), i
+
+
+@pytest.mark.parametrize("case", [0, 4, 5])
+def test_frame_fusion_matches_two_kernels(case, monkeypatch):
+    """The frame's fused C2R + R2C rows (Q = 4, 2 and 8 for the 4096-, 2048- and 8192-point images) give the
+    outputs and peaks of the same batch with the two kernels run one after the other, to FP32 rounding."""
+    sizes, n, nout, origin = BATCHES[case]
+    H.log(f"frame fusion against two kernels, {sizes} in {n} -> {nout}")
+    res = {}
+    for fused in (True, False):
+        if not fused:
+            monkeypatch.setenv("FASTFFT_NO_FRAME_FUSION", "1")
+        torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=17)
+        assert f"frame={FRAMES[(n, nout)]}" in ft.Describe("batch")
+        ft.FwdImageInvFFTBatch(img)
+        res[fused] = ([H.real_window(H.to_host(o), *nout) for o in outs], H.to_host(peaks).copy())
+    for a, b in zip(res[True][0], res[False][0]):
+        assert O.rel_l2(a, b) < 1e-6
+    assert np.array_equal(res[True][1], res[False][1])
 
 
 @pytest.mark.parametrize("case", [1, 2])
@@ -114,13 +142,13 @@ def test_batch_graph_replays(case):
         ft.FwdImageInvFFTBatch(img)
     for rep in range(3):
         for o in outs:
-            o.zero_()
+            o.fill_(float("nan"))
         peaks.fill_(-1)
         torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
         for i, o in enumerate(outs):
-            assert np.array_equal(H.to_host(o), want[i]), (rep, i)
+            assert np.array_equal(H.to_host(o), want[i], equal_nan=True), (rep, i)
         assert np.array_equal(H.to_host(peaks), want_pk) and np.array_equal(H.to_host(vals), want_v), rep
         ft.FwdImageInvFFTBatch(img)
         torch.cuda.synchronize()
@@ -150,13 +178,13 @@ def test_batch_parts_in_order_equal_the_call(case, monkeypatch):
     want = [H.to_host(o).copy() for o in outs]
     want_pk = H.to_host(peaks).copy()
     for o in outs:
-        o.zero_()
+        o.fill_(float("nan"))
     peaks.fill_(-1)
     for part in range(5):
         ft.RunBatchStage(part, img)
     torch.cuda.synchronize()
     for i, o in enumerate(outs):
-        assert np.array_equal(H.to_host(o), want[i]), i
+        assert np.array_equal(H.to_host(o), want[i], equal_nan=True), i
     assert np.array_equal(H.to_host(peaks), want_pk)
 
 
@@ -178,7 +206,7 @@ def test_batch_256_uneven_templates_4096():
     sizes = [(t.shape[1], t.shape[0]) for t in tmpls]
     ft = H.make_plan((384, 384, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
     tb = [H.to_dev(W.to_padded(t).reshape(-1)) for t in tmpls]
-    outs = [H.dev_zeros(ft.ReturnInvOutputMemorySize()) for _ in tmpls]
+    outs = [H.dev_poison(ft.ReturnInvOutputMemorySize()) for _ in tmpls]
     peaks = torch.full((2 * len(tmpls),), -1, dtype=torch.int32, device=d)
     vals = torch.zeros(len(tmpls), dtype=torch.float32, device=d)
     ft.SetBatch(tb, sizes, outs, peaks, vals)
@@ -200,3 +228,75 @@ def test_batch_256_uneven_templates_4096():
         if i % 16 == 0:
             got = H.real_window(H.to_host(outs[i]), m, m)
             assert O.rel_l2(got, H.real_window(H.to_host(o), m, m)) < TOL, i
+    # one template of each size and the last one against the FP64 oracle of the 4096^2 pipeline
+    mspec = H.complex_from_f32(H.to_host(img), (n // 2 + 1, n))
+    for i in (0, 96, 192, len(tmpls) - 1):
+        x, y = sizes[i]
+        got = H.real_window(H.to_host(outs[i]), m, m)
+        want = O.xcorr_2d(W.to_padded(tmpls[i]), x, mspec, n, n, m, m, -m // 2, -m // 2)
+        assert O.rel_l2(got, want) < TOL, i
+        wx, wy, _ = O.argmax_2d(want)
+        assert tuple(pk[i]) == (wx, wy), i
+
+
+def _planted_mixed(n, m, sizes, seed):
+    """Micrograph noise plus 0.5 x template i at a seeded lag inside the centred m x m window."""
+    from fastfft_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    tmpls = [W.masked_template(t, 0.42 * t, seed=seed + 1 + i) for i, t in enumerate(sizes)]
+    mic = rng.standard_normal((n, n)).astype(np.float32)
+    lags = []
+    for tp in tmpls:
+        t = tp.shape[0]
+        lx, ly = int(rng.integers(-m // 2 + 8, m // 2 - 8)), int(rng.integers(-m // 2 + 8, m // 2 - 8))
+        lags.append((lx, ly))
+        ys, xs = (np.arange(t) + ly) % n, (np.arange(t) + lx) % n
+        mic[np.ix_(ys, xs)] += 0.5 * tp
+    return mic, tmpls, lags
+
+
+@pytest.mark.parametrize("chunk", [0, 4])
+def test_headline_batch_4096_mixed_against_oracle(chunk, monkeypatch):
+    """The headline form: FP32 templates of 128^2, 256^2 and 384^2 (two of each, circular masks, each planted
+    once in a 4096^2 micrograph at a lag inside the window) correlated by one batched call on the reduced
+    1024^2 frame, 512^2 window of lags -256..255.  Every output is within 2e-6 (relative L2) of the FP64
+    oracle of the full 4096^2 pipeline, and every peak equals the oracle's and the planted lag.  chunk 4
+    runs the batch in two chunks of launches."""
+    from fastfft_amd import workloads as W
+
+    if chunk:
+        monkeypatch.setenv("FASTFFT_BATCH_CHUNK", str(chunk))
+    n, m = 4096, 512
+    H.log(f"headline batch 4096^2 x (128, 256, 384)^2 -> 512^2 against the FP64 oracle, chunk={chunk or 'tuned'}")
+    torch, d = H.torch_dev()
+    tsz = [128, 256, 384, 128, 256, 384]
+    mic, tmpls, lags = _planted_mixed(n, m, tsz, seed=71)
+    mft = H.make_plan((n, n, 1), (n, n, 1), (n, n, 1), (n, n, 1))
+    img = H.dev_zeros(mft.ReturnFwdOutputMemorySize())
+    img[:] = H.to_dev(W.to_padded(mic).reshape(-1))
+    mft.FwdFFT(img)
+    mspec = H.complex_from_f32(H.to_host(img), (n // 2 + 1, n))
+    ft = H.make_plan((384, 384, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0))
+    tb = [H.to_dev(W.to_padded(t).reshape(-1)) for t in tmpls]
+    outs = [H.dev_poison(ft.ReturnInvOutputMemorySize()) for _ in tmpls]
+    peaks = torch.full((2 * len(tmpls),), -1, dtype=torch.int32, device=d)
+    vals = torch.zeros(len(tmpls), dtype=torch.float32, device=d)
+    ft.SetBatch(tb, [(t, t) for t in tsz], outs, peaks, vals)
+    form = ft.Describe("batch").splitlines()[0]
+    assert "frame=1024x1024" in form and "precision=fp32" in form, form
+    if chunk:
+        assert f"chunk={chunk}" in form, form
+    ft.FwdImageInvFFTBatch(img)
+    pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
+    for i, tp in enumerate(tmpls):
+        t = tsz[i]
+        got = H.real_window(H.to_host(outs[i]), m, m)
+        want = O.xcorr_2d(W.to_padded(tp), t, mspec, n, n, m, m, -m // 2, -m // 2)
+        err = O.rel_l2(got, want)
+        print(f"template {i} ({t}^2): rel_l2={err:.3e} max_abs={O.max_abs(got, want):.3e}")
+        assert err < TOL, i
+        wx, wy, wv = O.argmax_2d(want)
+        assert tuple(pk[i]) == (wx, wy), i
+        assert (wx - m // 2, wy - m // 2) == lags[i], i
+        assert abs(float(vv[i]) - wv) <= 2e-6 * abs(wv), i

@@ -101,3 +101,100 @@ def test_fp16_batch_4096_512_templates():
     want = O.xcorr_2d(W.to_padded(tmpls[0].astype(np.float32)), t, _spec_f64(mic16, n), n, n, m, m, -m // 2, -m // 2,
                       post_scale=post)
     assert O.rel_l2(got, want) < TOL_F16
+
+
+def _half2_spectrum_host(buf, w, ny):
+    """The __half2 spectrum C[kx][ky] of a device FP16 buffer as complex128."""
+    a = H.to_host(buf)[: 2 * w * ny].astype(np.float64)
+    return (a[0::2] + 1j * a[1::2]).reshape(w, ny)
+
+
+@pytest.mark.parametrize("n,t,m,count", [(1024, 128, 256, 1), (4096, 384, 512, 3)])
+def test_fp32_templates_against_half2_spectrum(n, t, m, count):
+    """FourierTransformer<float, float, __half2, 2>: FP32 templates and outputs against a micrograph spectrum
+    stored as __half2 (the cisTEM practice of keeping the image spectrum in half precision).  The oracle
+    correlates the same FP32 templates with the same __half2-rounded spectrum, so the FP32 tolerance
+    applies; single correlations and a batch (on the reduced frame at 4096^2) both run."""
+    from fastfft_amd import workloads as W
+
+    H.log(f"<float, float, __half2, 2> correlation {n}^2 x {t}^2 -> {m}^2, {count} templates")
+    torch, d, tmpls16, lags, mic16, mbuf = _case(n, t, m, 0.43 * t, count, seed=3 * n + t)
+    spec = _half2_spectrum_host(mbuf, n // 2 + 1, n)
+    # the planted templates were FP16-rounded; the FP32 templates here are those same values
+    tmpls = [tp.astype(np.float32) for tp in tmpls16]
+    ft = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0), input_type="f32",
+                     image_type="c32h")
+    post = 1.0 / (t * t)
+    outs = []
+    for tp in tmpls:
+        a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()))
+        tpp = W.to_padded(tp).reshape(-1)
+        a[: tpp.size] = H.to_dev(tpp)
+        out = H.dev_poison(ft.ReturnInvOutputMemorySize())
+        peak = torch.zeros(2, dtype=torch.int32, device=d)
+        val = torch.zeros(1, dtype=torch.float32, device=d)
+        ft.FwdImageInvFFTArgMax(a, mbuf, out, peak, val, post_op=H.F.Scale(post))
+        outs.append((out, peak))
+    bft = H.make_plan((t, t, 1), (n, n, 1), (n, n, 1), (m, m, 1), origin=(-m // 2, -m // 2, 0), input_type="f32",
+                      image_type="c32h")
+    tb = [H.to_dev(W.to_padded(tp).reshape(-1)) for tp in tmpls]
+    bouts = [H.dev_poison(bft.ReturnInvOutputMemorySize()) for _ in tmpls]
+    bpk = torch.full((2 * count,), -1, dtype=torch.int32, device=d)
+    bval = torch.zeros(count, dtype=torch.float32, device=d)
+    bft.SetBatch(tb, [(t, t)] * count, bouts, bpk, bval)
+    bft.FwdImageInvFFTBatch(mbuf, post_op=H.F.Scale(post))
+    bpk = H.to_host(bpk).reshape(-1, 2)
+    for i, tp in enumerate(tmpls):
+        want = O.xcorr_2d(W.to_padded(tp), t, spec, n, n, m, m, -m // 2, -m // 2, post_scale=post)
+        wx, wy, _ = O.argmax_2d(want)
+        got = H.real_window(H.to_host(outs[i][0]), m, m)
+        bgot = H.real_window(H.to_host(bouts[i]), m, m)
+        print(f"template {i}: single rel_l2={O.rel_l2(got, want):.3e} batch rel_l2={O.rel_l2(bgot, want):.3e}")
+        assert O.rel_l2(got, want) < 2e-6, i
+        assert O.rel_l2(bgot, want) < 2e-6, i
+        assert tuple(H.to_host(outs[i][1])) == (wx, wy) == tuple(bpk[i]), i
+        assert (wx - m // 2, wy - m // 2) == tuple(lags[i]), i
+
+
+def _spec_host_3d(buf, n):
+    a = H.to_host(buf)[: 2 * n[2] * (n[0] // 2 + 1) * n[1]].astype(np.float64)
+    return (a[0::2] + 1j * a[1::2]).reshape(n[2], n[0] // 2 + 1, n[1])
+
+
+@pytest.mark.parametrize("tin,n,nout,origin", [((8, 8, 8), (32, 32, 32), (16, 16, 16), (-8, -8, -8)),
+                                                ((16, 12, 10), (64, 32, 32), (64, 32, 32), (0, 0, 0))])
+def test_half_3d_fwd_and_xcorr(tin, n, nout, origin):
+    """FourierTransformer<float, __half, __half2, 3>: FP16 volumes, the padded 3-D forward transform stored
+    as __half2, and the 3-D correlation of an FP16 template against that __half2 spectrum with an FP16
+    trimmed output.  The oracle takes the same FP16-rounded inputs (and, for the correlation, the
+    __half2 spectrum the library stored), so the error is that of the FP16 outputs."""
+    H.log(f"<float, __half, __half2, 3> fwd and xcorr {tin} in {n} -> {nout} at {origin}")
+    rng = np.random.default_rng(sum(n) + sum(tin))
+    img = rng.standard_normal(n[::-1]).astype(np.float16)            # [z][y][x]
+    tmpl = rng.standard_normal(tin[::-1]).astype(np.float16)
+    ntot = n[0] * n[1] * n[2]
+    # image spectrum: FP16 volume -> __half2 spectrum, pre-scaled so that FP16 holds it
+    mft = H.make_plan(n, n, n, n, input_type="f16", image_type="c32h", rank=3)
+    ib = H.to_dev(H.padded_real(img, n[0], np.float16).reshape(-1))
+    spec_buf = H.dev_zeros(mft.ReturnFwdOutputMemorySize(), "f16")
+    pre = 1.0 / np.sqrt(ntot)
+    mft.FwdFFT(ib, spec_buf, pre_op=H.F.Scale(pre))
+    got_spec = _spec_host_3d(spec_buf, n)
+    want_spec = O.fwd_3d(H.padded_real(img.astype(np.float32), n[0]), n[0], *n) * pre
+    err_spec = O.rel_l2(got_spec, want_spec)
+    print(f"fwd 3d __half2 spectrum rel_l2={err_spec:.3e}")
+    assert err_spec < TOL_F16
+    # correlation: FP16 template padded to n, against the stored __half2 spectrum, FP16 output
+    ft = H.make_plan(tin, n, n, nout, origin=origin, input_type="f16", image_type="c32h", rank=3)
+    a = H.dev_zeros(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()), "f16")
+    tp = H.padded_real(tmpl, tin[0], np.float16).reshape(-1)
+    a[: tp.size] = H.to_dev(tp)
+    out = H.dev_poison(ft.ReturnInvOutputMemorySize(), "f16")
+    post = 1.0 / (ntot * np.sqrt(tin[0] * tin[1] * tin[2]))
+    ft.FwdImageInvFFT(a, spec_buf, out, post_op=H.F.Scale(post))
+    got = H.real_window(H.to_host(out).astype(np.float32), *nout)
+    spec_t = O.fwd_3d(H.padded_real(tmpl.astype(np.float32), tin[0]), tin[0], *n)
+    want = O.inv_3d(got_spec * np.conj(spec_t), *n, *nout, *origin) * post
+    err = O.rel_l2(got, want)
+    print(f"xcorr 3d FP16 output rel_l2={err:.3e}")
+    assert err < TOL_F16

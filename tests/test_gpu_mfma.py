@@ -2,7 +2,7 @@
 
 The correlation pass of each 1024 x 1024 frame runs its DFT-32 stages as hi + lo BF16 matrix products
 with FP32 accumulation; every other pass stays FP32.  Outputs are compared with the FP64 oracle of the
-full 4096 x 4096 pipeline (relative L2 within 2e-5), and every peak must equal the FP32 path's, the
+full 4096 x 4096 pipeline (relative L2 within 2e-5; the FP32 form of the same batch within 2e-6), and every peak must equal the FP32 path's, the
 oracle's and the planted lag.  The FP16 I/O case has the shape of 512 x 512 FP16 templates (radius-220
 masks) against a 4096 x 4096 FP16 micrograph; there FP16 storage sets the error bound (2e-3, as for the
 FP32 path).
@@ -17,6 +17,7 @@ from . import gpu_helpers as H
 pytestmark = pytest.mark.gpu
 
 TOL_BF16 = 2e-5
+TOL_F32 = 2e-6
 TOL_F16 = 2e-3
 
 
@@ -37,7 +38,7 @@ def _planted(n, t, m, radius, count, seed):
 
 
 def _batch(ft, tbufs, t, n_out_elems, precision, torch, d):
-    outs = [H.dev_zeros(n_out_elems) for _ in tbufs]
+    outs = [H.dev_poison(n_out_elems) for _ in tbufs]
     peaks = torch.full((2 * len(tbufs),), -1, dtype=torch.int32, device=d)
     vals = torch.zeros(len(tbufs), dtype=torch.float32, device=d)
     ft.SetPrecision(precision)
@@ -74,6 +75,7 @@ def test_split_bf16_batch_4096_384_against_oracle():
         err32 = O.rel_l2(H.real_window(H.to_host(outs32[i]), m, m), want)
         print(f"template {i}: split-BF16 rel_l2={err:.3e} (FP32 path {err32:.3e})")
         assert err < TOL_BF16, i
+        assert err32 < TOL_F32, i
         wx, wy, _ = O.argmax_2d(want)
         assert tuple(pk[i]) == (wx, wy) == tuple(pk32[i]), i
         assert (wx - m // 2, wy - m // 2) == lags[i], i
@@ -87,12 +89,12 @@ def test_split_bf16_batch_4096_384_against_oracle():
         ft.FwdImageInvFFTBatch(mbuf)
     for rep in range(2):
         for o in outs:
-            o.zero_()
+            o.fill_(float("nan"))
         torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
         for i, o in enumerate(outs):
-            assert np.array_equal(H.to_host(o), want_outs[i]), (rep, i)
+            assert np.array_equal(H.to_host(o), want_outs[i], equal_nan=True), (rep, i)
 
 
 def test_split_bf16_fp16_io_512_templates():

@@ -2,8 +2,8 @@
 
 With statistics set, a batched call updates per-pixel maps in place of writing the correlation maps: the
 maximum over templates, the template id of its first occurrence, the sum and the sum of squares.  The
-maps are checked against the same batch's own correlation maps (the batch run without statistics),
-reduced on the host in template order in FP64: maximum and sums within FP32 rounding, the template id
+maps are checked against the FP64 oracle's correlation maps and against the same batch's own maps (the
+batch run without statistics), both reduced on the host in template order in FP64: maximum and sums within FP32 rounding, the template id
 exactly wherever the two largest values are further apart than that rounding.  Also covered:
 accumulation over two calls and over several chunks, graph replays, the reset, untouched outputs, and the
 batches the statistics form does not take.
@@ -84,6 +84,26 @@ def test_batch_stats_match_the_maps(case, mode, monkeypatch):
     assert "stats=on" not in ft.Describe("batch")
     again = _maps(ft, img, outs, nout)
     assert np.array_equal(again, maps)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_batch_stats_against_oracle_maps(case):
+    """The statistics of one call against the FP64 oracle: every template's correlation map from the
+    oracle of the full N-frame pipeline, reduced on the host in FP64 (maximum, first template reaching it,
+    sum, sum of squares).  Case 0 is the 4096^2 spectrum on the reduced 1024^2 frame with 128^2, 256^2
+    and 384^2 templates and the centred 512^2 window."""
+    from oracle import oracle as O
+
+    sizes, n, nout, origin = BATCHES[case]
+    H.log(f"batch stats {sizes} in {n} -> {nout} at {origin} against oracle maps")
+    torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=51 + case)
+    mip, best, s, sq = _stats_tensors(torch, nout)
+    ft.SetBatchStats(mip, best, s, sq, first_id=3)
+    ft.ResetBatchStats()
+    ft.FwdImageInvFFTBatch(img)
+    want = np.stack([O.xcorr_2d(H.padded_real(t, x), x, spec, n[0], n[1], nout[0], nout[1], *origin)
+                     for t, (x, y) in zip(hosts, sizes)])  # [t][y][x], FP64
+    _check([want], [3], mip, best, s, sq, nout)
 
 
 @pytest.mark.parametrize("case", CASES)
