@@ -561,7 +561,8 @@ static bool batch_mixed(fastfft_plan p) {
         return false;
     const Stage &f = p->sub_fwd[0], &x = p->sub_xc[1];
     return f.kind == StageKind::fwd_real && (f.dst == Buf::ws_a || f.dst == Buf::ws_b) && f.om.s1 == 1 &&
-           f.om.n2 == 1 && f.om.es == x.gm.s1 && f.N == x.N && f.nout == x.nlines && xcorr_toeplitz_covers(x);
+           f.om.n2 == 1 && f.om.es == x.gm.s1 && f.N == x.N && f.nlines == x.N && f.nout == x.nlines &&
+           xcorr_toeplitz_covers(x);
 }
 
 // Tuning-cache key of a batch: device architecture, precision, element types, the plan's
@@ -1050,6 +1051,10 @@ fastfft_status fastfft_set_precision(fastfft_plan plan, int precision) {
         return rc;
     if ( precision != FASTFFT_PRECISION_FP32 && precision != FASTFFT_PRECISION_SPLIT_BF16 )
         return fail(FASTFFT_INVALID_ARGUMENT, "precision must be FASTFFT_PRECISION_FP32 or FASTFFT_PRECISION_SPLIT_BF16");
+    // the batch's captured graph holds the other precision's kernels (and, on a frame, its
+    // frame passes): it is captured again at the next call
+    if ( plan->precision != precision )
+        free_batch_graph(plan);
     plan->precision = precision;
     return ok( );
 }
