@@ -98,6 +98,7 @@ def lib():
         "fastfft_inv": [P, V, V, _Op, _Op],
         "fastfft_fwd_image_inv": [P, V, V, V, _Op, _Op, _Op],
         "fastfft_fwd_image_inv_argmax": [P, V, V, V, _Op, _Op, _Op, V, V],
+        "fastfft_fwd_image_inv_argmax_noalloc": [P, V, V, V, _Op, _Op, _Op, V, V],
         "fastfft_set_batch": [P, ctypes.c_int, V, V, V, V, V],
         "fastfft_fwd_image_inv_batch": [P, V, _Op, _Op, _Op],
         "fastfft_set_batch_stats": [P, V, V, V, V, I],
@@ -260,6 +261,16 @@ class FourierTransformer:
         _check(lib().fastfft_fwd_image_inv_argmax(self._h, _ptr(input_ptr), _ptr(image_to_search), _ptr(output_ptr),
                                                   pre_op or Noop(), intra_op or ConjMul(), post_op or Noop(),
                                                   _ptr(peak_xy), _ptr(value)))
+
+    def FwdImageInvFFTArgMaxNoAlloc(self, input_ptr, image_to_search, output_ptr, peak_xy, value, pre_op=None,
+                                    intra_op=None, post_op=None):
+        """Extension: FwdImageInvFFTArgMax whose intermediates live in the library's static device scratch
+        (32 MiB per GPU): the call allocates no device memory.  Calls that overlap on one GPU race; plans
+        that need more scratch raise FastFFTError (UNSUPPORTED).  2-D plans only."""
+        self._bind_torch_stream()
+        _check(lib().fastfft_fwd_image_inv_argmax_noalloc(self._h, _ptr(input_ptr), _ptr(image_to_search),
+                                                          _ptr(output_ptr), pre_op or Noop(), intra_op or ConjMul(),
+                                                          post_op or Noop(), _ptr(peak_xy), _ptr(value)))
 
     def SetBatch(self, templates, sizes, outputs, peaks_xy=None, values=None) -> None:
         """Extension: register templates (device buffers or pointers), their (x, y) sizes and output buffers

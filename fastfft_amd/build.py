@@ -23,7 +23,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
-SOURCES = ["planner.cpp", "capi.cpp", "launch_fwd.hip", "launch_inv.hip", "launch_xcorr.hip", "launch_tile.hip", "misc.hip",
+SOURCES = ["planner.cpp", "capi.cpp", "launch_fwd.hip", "launch_inv.hip", "launch_xcorr.hip", "launch_tile.hip", "misc.hip", "scratch.hip",
            "tuning.cpp"]
 HEADERS = ["fft_core.hpp", "line_kernels.hpp", "launch.hpp", "launch_common.hpp", "plan.hpp", "wave_fft.hpp",
            "xcorr_col.hpp", "rows_wave.hpp", "items.hpp", "full_wave.hpp", "window_rows.hpp", "xcorr_toeplitz.hpp", "tile_lines.hpp", "rows_packed.hpp", "tuning.hpp"]

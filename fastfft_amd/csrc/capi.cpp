@@ -62,7 +62,8 @@ hipError_t dev_alloc(void** p, size_t bytes) {
 constexpr size_t kMaxLdsBytes = 160 * 1024; // gfx950 LDS per CU
 constexpr int    kMaxDevices  = 64;
 
-// Per-device FP32 twiddle table, generated once in FP64 and kept until exit.
+// Per-device FP32 twiddle table (a static device array), generated once in FP64 at the
+// first transform on the device and kept until exit.
 std::mutex g_tw_mutex;
 float2*    g_tw[kMaxDevices] = {nullptr};
 
@@ -76,16 +77,14 @@ hipError_t twiddles_for_current_device(float2** out, hipStream_t stream) {
     std::lock_guard<std::mutex> lock(g_tw_mutex);
     if ( ! g_tw[dev] ) {
         float2* tw = nullptr;
-        e          = hipMalloc(&tw, sizeof(float2) * kTwiddleTableN);
+        e          = twiddle_table_address(&tw);
         if ( e != hipSuccess )
             return e;
         e = launch_twiddle_init(tw, stream);
         if ( e == hipSuccess )
             e = hipStreamSynchronize(stream);
-        if ( e != hipSuccess ) {
-            (void)hipFree(tw);
+        if ( e != hipSuccess )
             return e;
-        }
         g_tw[dev] = tw;
     }
     *out = g_tw[dev];
@@ -1152,6 +1151,73 @@ fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const v
         const Dims& o = plan->st.inv_out;
         hipError_t  e = launch_argmax(dst, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, o.x, o.y,
                                       2 * o.w, a.pv, a.pi, value, peak_xy, plan->stream);
+        if ( e != hipSuccess )
+            return hip_fail(e, "argmax");
+    }
+    return ok( );
+}
+
+fastfft_status fastfft_fwd_image_inv_argmax_noalloc(fastfft_plan plan, void* in, const void* image_to_search, void* out,
+                                                    fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op,
+                                                    int* peak_xy, float* value) {
+    if ( auto rc = check_plan(plan) )
+        return rc;
+    if ( ! in || ! image_to_search || ! peak_xy || ! value )
+        return fail(FASTFFT_INVALID_ARGUMENT, "null input, image_to_search, peak_xy or value pointer");
+    if ( auto rc = op_kind_check(pre_op, true, false, "the FwdImageInvFFT pre op") )
+        return rc;
+    if ( intra_op.kind != FASTFFT_OP_CONJ_MUL && intra_op.kind != FASTFFT_OP_CONJ_MUL_THEN_SCALE )
+        return fail(FASTFFT_INVALID_ARGUMENT, "FwdImageInvFFT needs an intra op (CONJ_MUL or CONJ_MUL_THEN_SCALE)");
+    if ( auto rc = op_kind_check(post_op, true, false, "the FwdImageInvFFT post op") )
+        return rc;
+    if ( auto rc = ensure_validated(plan) )
+        return rc;
+    if ( plan->st.transform_dimension != 2 )
+        return fail(FASTFFT_UNSUPPORTED, "FwdImageInvFFT with argmax is implemented for 2D plans");
+    const std::vector<Stage>* sp = nullptr;
+    if ( auto rc = get_stages(plan, 2, &sp) )
+        return rc;
+    // scratch layout: the stage list's two intermediates, then the argmax row keys, the
+    // partial values and indices of the two-kernel epilogue (each part 256-byte aligned)
+    size_t a = 0, b = 0;
+    stage_list_workspace(*sp, a, b);
+    auto         up    = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t off_b = up(sizeof(float2) * a);
+    const size_t off_k = off_b + up(sizeof(float2) * b);
+    const size_t off_v = off_k + up(sizeof(unsigned long long) * kArgmaxPartials);
+    const size_t off_i = off_v + up(sizeof(float) * kArgmaxPartials);
+    const size_t need  = off_i + up(sizeof(int) * kArgmaxPartials);
+    if ( need > kStaticScratchBytes )
+        return fail(FASTFFT_UNSUPPORTED, "fwd_image_inv_argmax_noalloc: this plan needs " + std::to_string(need) +
+                                             " bytes of scratch, more than the " + std::to_string(kStaticScratchBytes) +
+                                             " bytes of the library's static scratch (use fastfft_fwd_image_inv_argmax)");
+    // the twiddle table is a static device array as well: the first call on a device fills it
+    float2*    tw = nullptr;
+    hipError_t e  = twiddles_for_current_device(&tw, plan->stream);
+    if ( e != hipSuccess )
+        return hip_fail(e, "twiddle table");
+    void* base = nullptr;
+    e          = static_scratch_for_current_device(&base);
+    if ( e != hipSuccess )
+        return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    char*       sc = static_cast<char*>(base);
+    ArgmaxSlots slots;
+    slots.keys  = reinterpret_cast<unsigned long long*>(sc + off_k);
+    slots.value = value;
+    slots.xy    = peak_xy;
+    void* dst   = out ? out : in;
+    bool  fused = false;
+    if ( auto rc = run_stage_list(plan, *sp, "xcorr", in, dst, image_to_search, op_scale(pre_op), op_scale(intra_op),
+                                  op_scale(post_op), 0, -1, &slots, &fused, reinterpret_cast<float2*>(sc),
+                                  reinterpret_cast<float2*>(sc + off_b), tw) )
+        return rc;
+    plan->last_buffer     = dst;
+    plan->stage_completed = 7;
+    if ( ! fused ) {
+        const Dims& o = plan->st.inv_out;
+        e             = launch_argmax(dst, plan->st.input_type == FASTFFT_INPUT_F16 ? RealT::f16 : RealT::f32, o.x, o.y,
+                                      2 * o.w, reinterpret_cast<float*>(sc + off_v), reinterpret_cast<int*>(sc + off_i),
+                                      value, peak_xy, plan->stream);
         if ( e != hipSuccess )
             return hip_fail(e, "argmax");
     }

@@ -108,6 +108,15 @@ constexpr int kTwiddleTableN = 3 * 8192;
 
 // Fills the twiddle table (FP64 on the GPU, rounded to FP32).
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream);
+// Device address of the current device's twiddle table (a static device array of
+// kTwiddleTableN entries; launch_twiddle_init fills it).
+hipError_t twiddle_table_address(float2** out);
+
+// Device scratch of the allocation-free correlation entry point: a static device array
+// of kStaticScratchBytes per GPU (scratch.hip).  *out receives the current device's
+// copy; the address is looked up once per device and kept.
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+hipError_t       static_scratch_for_current_device(void** out);
 
 // Argmax of a real [ny][pitch] image (first maximum in row-major order).  Two
 // kernels; partial_v / partial_i hold kArgmaxPartials entries of device scratch;

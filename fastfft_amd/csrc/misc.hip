@@ -27,6 +27,18 @@ __global__ void k_twiddle_init(float2* __restrict__ tw) {
     tw[k] = make_float2((float)c, (float)(-s));
 }
 
+// The table itself: a static device array (one copy per GPU, kept until the process
+// ends), so that no call allocates it.  Its device address comes from
+// hipGetSymbolAddress, called with the device current.
+static __device__ __align__(256) float2 twiddle_table[kTwiddleTableN];
+
+hipError_t twiddle_table_address(float2** out) {
+    void*            p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(twiddle_table));
+    *out               = e == hipSuccess ? static_cast<float2*>(p) : nullptr;
+    return e;
+}
+
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream) {
     k_twiddle_init<<<dim3((kTwiddleTableN + 255) / 256), dim3(256), 0, stream>>>(tw);
     return hipGetLastError( );

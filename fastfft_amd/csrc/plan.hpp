@@ -94,6 +94,8 @@ PlanError build_stages(const PlanState& p, const std::string& op, std::vector<St
 
 // Workspace (ws_a, ws_b) complex-float counts needed by all operations of p.
 void workspace_elems(const PlanState& p, size_t& ws_a, size_t& ws_b);
+// Raises ws_a / ws_b to the complex-float counts one stage list writes.
+void stage_list_workspace(const std::vector<Stage>& st, size_t& ws_a, size_t& ws_b);
 
 // Sub-transform size for a line of N with L nonzero inputs and M kept outputs.
 int pick_sub_size(int N, int L, int M);

@@ -483,6 +483,17 @@ static size_t extent(const HostLineMap& m, int nlines, int nelem) {
     return (size_t)(std::max(b, b2) + (long)(nelem - 1) * m.es + 1);
 }
 
+void stage_list_workspace(const std::vector<Stage>& st, size_t& ws_a, size_t& ws_b) {
+    for ( const Stage& s : st ) {
+        int    nw = s.kind == StageKind::fwd_real || s.kind == StageKind::fwd_cpx ? s.nout : s.M;
+        size_t ex = extent(s.om, s.nlines, nw);
+        if ( s.dst == Buf::ws_a )
+            ws_a = std::max(ws_a, ex);
+        if ( s.dst == Buf::ws_b )
+            ws_b = std::max(ws_b, ex);
+    }
+}
+
 void workspace_elems(const PlanState& p, size_t& ws_a, size_t& ws_b) {
     ws_a = ws_b = 0;
     const char* ops[3] = {"fwd", "inv", "xcorr"};
@@ -490,14 +501,7 @@ void workspace_elems(const PlanState& p, size_t& ws_a, size_t& ws_b) {
         std::vector<Stage> st;
         if ( build_stages(p, op, st).code != 0 )
             continue;
-        for ( const Stage& s : st ) {
-            int    nw = s.kind == StageKind::fwd_real || s.kind == StageKind::fwd_cpx ? s.nout : s.M;
-            size_t ex = extent(s.om, s.nlines, nw);
-            if ( s.dst == Buf::ws_a )
-                ws_a = std::max(ws_a, ex);
-            if ( s.dst == Buf::ws_b )
-                ws_b = std::max(ws_b, ex);
-        }
+        stage_list_workspace(st, ws_a, ws_b);
     }
 }
 

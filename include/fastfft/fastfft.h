@@ -133,6 +133,23 @@ fastfft_status fastfft_fwd_image_inv_argmax(fastfft_plan plan, void* in, const v
                                             fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op, int* peak_xy,
                                             float* value);
 
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): fastfft_fwd_image_inv_argmax with the same
+ * arguments and results, whose intermediates and argmax slots live in the
+ * library's static device scratch (32 MiB per GPU, held until the process
+ * ends) instead of the plan's workspace.  It calls no hipMalloc, hipFree or
+ * memset; the caller passes the inputs and outputs.  Plans whose correlation
+ * needs more scratch than that (4096^2 with a 512^2 window and a template up
+ * to 512^2 needs about 17 MB) get FASTFFT_UNSUPPORTED.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap in time on one GPU (on
+ * different streams or host threads) race.  The first call on a device fills
+ * the device's twiddle table (one kernel and a stream synchronisation), so it
+ * is not made under graph capture; later calls can be captured.  2-D plans only. */
+fastfft_status fastfft_fwd_image_inv_argmax_noalloc(fastfft_plan plan, void* in, const void* image_to_search, void* out,
+                                                    fastfft_op pre_op, fastfft_op intra_op, fastfft_op post_op,
+                                                    int* peak_xy, float* value);
+
 /* Extension (batched correlation): registers n templates for
  * fastfft_fwd_image_inv_batch.  Template i is a real sizes_xy[2i] x sizes_xy[2i+1]
  * device array, FFTW padded to its own pitch 2*(x/2+1), no larger than the

@@ -206,3 +206,21 @@ def test_set_precision_values():
     with pytest.raises(F.FastFFTError) as e:
         ft.SetPrecision(7)
     assert e.value.status == F.INVALID_ARGUMENT
+
+
+def test_noalloc_rejects_plans_beyond_static_scratch():
+    """The allocation-free correlation checks its scratch need on the host, before any device call:
+    an 8192^2 correlation with a 1024^2 template and a 2048^2 window needs about 100 MB, over the 32 MiB."""
+    ft = F.FourierTransformer()
+    ft.SetForwardFFTPlan(1024, 1024, 1, 8192, 8192, 1)
+    ft.SetInverseFFTPlan(8192, 8192, 1, 2048, 2048, 1)
+    with pytest.raises(F.FastFFTError) as e:
+        ft.FwdImageInvFFTArgMaxNoAlloc(1, 2, 3, 4, 5)
+    assert e.value.status == F.UNSUPPORTED and "33554432 bytes" in str(e.value)
+    # a 3-D plan is refused as by the allocating entry point
+    f3 = F.FourierTransformer("f32", "c64", 3)
+    f3.SetForwardFFTPlan(16, 16, 16, 64, 64, 64)
+    f3.SetInverseFFTPlan(64, 64, 64, 64, 64, 64)
+    with pytest.raises(F.FastFFTError) as e:
+        f3.FwdImageInvFFTArgMaxNoAlloc(1, 2, 3, 4, 5)
+    assert e.value.status == F.UNSUPPORTED
