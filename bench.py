@@ -57,10 +57,10 @@ if ROOT not in sys.path:
 N, T, M = 4096, 384, 512
 HBM_PEAK_GBS = 8000.0
 # the split-BF16 correlation pass of the 1024-point frame: per column a Toeplitz product of 512 output
-# rows by K = 2 x 384 interleaved template values for every template, as 2 (real and imaginary part)
-# x 3 (hi*hi, hi*lo, lo*hi) products; v_mfma_f32_32x32x16_bf16 instructions per template column
+# rows by K = 384 template rows for every template, as 3 real products (Gauss's form of the complex
+# product) x 3 split terms (hi*hi, hi*lo, lo*hi); v_mfma_f32_32x32x16_bf16 instructions per template column
 MFMA_KERNEL = "k_xcorr_toeplitz"
-MFMA_PER_COLUMN = 2 * 3 * (M // 32) * (2 * T // 16) // 32
+MFMA_PER_COLUMN = 3 * 3 * (M // 32) * (T // 16) / 32
 TRAFFIC_PROFILE = "profiles/xcorr_col_frame1024_batch_hbm_traffic.md"
 
 
@@ -73,6 +73,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-batch", type=int, default=10, help="correlations per captured graph (single form)")
+    ap.add_argument("--only", default="all",
+                    help="comma-separated sections besides the headline batch (stats, split_bf16, uneven, config5, "
+                         "config4, single, cpu), e.g. for a short profiled run; default all")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -91,6 +94,10 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     B, K = args.batch, args.steps
+    only = {x.strip() for x in args.only.split(",") if x.strip()}
+
+    def want(section):
+        return "all" in only or section in only
 
     mic, tmpl, lags = W.xcorr_config(n=N, tsize=T, radius=160.0, out=M)
 
@@ -193,194 +200,199 @@ def main() -> int:
     bft.SetStream(torch.cuda.current_stream())
 
     # ---------------------------------------------------------------- template-matching statistics form:
-    # the same batch updating per-pixel maximum / its template / sum / sum of squares in place of the maps
-    bft.FwdImageInvFFTBatch(spec, intra_op=conj)  # the maps of every template, for the comparison
-    torch.cuda.synchronize()
-    pitch = 2 * (M // 2 + 1)
-    maps = torch.stack([o[: pitch * M].view(M, pitch)[:, :M] for o in outs])
-    want_mip, want_best = maps.max(dim=0)
-    st = [torch.zeros(M * M, dtype=dt, device=dev) for dt in (torch.float32, torch.int32, torch.float32, torch.float32)]
-    bft.SetBatchStats(*st, first_id=0)
-    bft.SetStream(cap)
-    g_st = capture(lambda: (bft.ResetBatchStats(), bft.FwdImageInvFFTBatch(spec, intra_op=conj)), 1)
-    st_reps = max(2, K // 4)
-    st_ms = time_graph(g_st, st_reps, warm=3) / st_reps
-    bft.SetStream(torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    scale = float(maps.abs().max())
-    top2 = torch.topk(maps, 2, dim=0).values
-    clear = (top2[0] - top2[1]) > 1e-5 * scale
-    stats_2dtm = {
-        "value": round(B * 1000.0 / st_ms, 1),
-        "unit": "template correlations/s",
-        "ms_per_call": round(st_ms, 5),
-        "form": bft.Describe("batch").splitlines()[0],
-        "mip_max_err_rel": float(f"{float((st[0].view(M, M) - want_mip).abs().max()) / scale:.3e}"),
-        "best_agree_frac_clear_pixels": round(float((st[1].view(M, M) == want_best)[clear].float().mean()), 6),
-        "sum_max_err_rel": float(f"{float((st[2].view(M, M) - maps.sum(0)).abs().max()) / float(maps.abs().sum(0).max()):.3e}"),
-        "timing": f"HIP graph of ResetBatchStats + one FwdImageInvFFTBatch call in statistics form replayed {st_reps} times",
-    }
-    bft.SetBatchStats()
-    del maps, want_mip, want_best, st, top2, clear
+    if want("stats"):
+        # the same batch updating per-pixel maximum / its template / sum / sum of squares in place of the maps
+        bft.FwdImageInvFFTBatch(spec, intra_op=conj)  # the maps of every template, for the comparison
+        torch.cuda.synchronize()
+        pitch = 2 * (M // 2 + 1)
+        maps = torch.stack([o[: pitch * M].view(M, pitch)[:, :M] for o in outs])
+        want_mip, want_best = maps.max(dim=0)
+        st = [torch.zeros(M * M, dtype=dt, device=dev) for dt in (torch.float32, torch.int32, torch.float32, torch.float32)]
+        bft.SetBatchStats(*st, first_id=0)
+        bft.SetStream(cap)
+        g_st = capture(lambda: (bft.ResetBatchStats(), bft.FwdImageInvFFTBatch(spec, intra_op=conj)), 1)
+        st_reps = max(2, K // 4)
+        st_ms = time_graph(g_st, st_reps, warm=3) / st_reps
+        bft.SetStream(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        scale = float(maps.abs().max())
+        top2 = torch.topk(maps, 2, dim=0).values
+        clear = (top2[0] - top2[1]) > 1e-5 * scale
+        stats_2dtm = {
+            "value": round(B * 1000.0 / st_ms, 1),
+            "unit": "template correlations/s",
+            "ms_per_call": round(st_ms, 5),
+            "form": bft.Describe("batch").splitlines()[0],
+            "mip_max_err_rel": float(f"{float((st[0].view(M, M) - want_mip).abs().max()) / scale:.3e}"),
+            "best_agree_frac_clear_pixels": round(float((st[1].view(M, M) == want_best)[clear].float().mean()), 6),
+            "sum_max_err_rel": float(f"{float((st[2].view(M, M) - maps.sum(0)).abs().max()) / float(maps.abs().sum(0).max()):.3e}"),
+            "timing": f"HIP graph of ResetBatchStats + one FwdImageInvFFTBatch call in statistics form replayed {st_reps} times",
+        }
+        bft.SetBatchStats()
+        del maps, want_mip, want_best, st, top2, clear
 
     # ---------------------------------------------------------------- split-BF16 form
-    sft, souts, spk, s_ms, _ = batch_form("split_bf16")
-    s_out0 = souts[0].cpu().numpy()
-    s_peaks_same = bool(np.array_equal(spk.cpu().numpy(), b_pk_all))
-    s_head = sft.Describe("batch").splitlines()[0]
-    s_chunk = int(re.search(r"chunk=(\d+)", s_head).group(1))
-    g_sx = capture(lambda: sft.RunBatchStage(2, spec, intra_op=conj), 1)
-    s_xc_ms = time_graph(g_sx, reps) / reps / ((B + s_chunk - 1) // s_chunk)
-    # matrix-core work of the split-BF16 correlation pass per launch: MFMA_PER_COLUMN
-    # v_mfma_f32_32x32x16_bf16 (32 x 32 x 16 x 2 flops each) per template column, over the
-    # fx / 2 column workgroups (the DC and Nyquist columns share one) and whole groups of 32 templates
-    s_mfma_flops = (s_chunk + 31) // 32 * 32 * (wf - 1) * MFMA_PER_COLUMN * 32 * 32 * 16 * 2
-    s_mfma_tflops = s_mfma_flops / (s_xc_ms * 1e-3) / 1e12
-    sft.SetStream(torch.cuda.current_stream())
+    if want("split_bf16"):
+        sft, souts, spk, s_ms, _ = batch_form("split_bf16")
+        s_out0 = souts[0].cpu().numpy()
+        s_peaks_same = bool(np.array_equal(spk.cpu().numpy(), b_pk_all))
+        s_head = sft.Describe("batch").splitlines()[0]
+        s_chunk = int(re.search(r"chunk=(\d+)", s_head).group(1))
+        g_sx = capture(lambda: sft.RunBatchStage(2, spec, intra_op=conj), 1)
+        s_xc_ms = time_graph(g_sx, reps) / reps / ((B + s_chunk - 1) // s_chunk)
+        # matrix-core work of the split-BF16 correlation pass per launch: MFMA_PER_COLUMN
+        # v_mfma_f32_32x32x16_bf16 (32 x 32 x 16 x 2 flops each) per template column, over the
+        # fx / 2 column workgroups (the DC and Nyquist columns share one) and whole groups of 32 templates
+        s_mfma_flops = round((s_chunk + 31) // 32 * 32 * (wf - 1) * MFMA_PER_COLUMN * 32 * 32 * 16 * 2)
+        s_mfma_tflops = s_mfma_flops / (s_xc_ms * 1e-3) / 1e12
+        sft.SetStream(torch.cuda.current_stream())
 
     # ---------------------------------------------------------------- uneven batch of 256 (96 x 128^2,
-    # 96 x 256^2, 64 x 384^2), one call per step
-    utm = W.batched_templates()
-    usizes = [(t.shape[1], t.shape[0]) for t in utm]
-    ubufs = [torch.from_numpy(W.to_padded(t).reshape(-1)).to(dev) for t in utm]
-    uft = plan()
-    uouts = [torch.zeros(uft.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev) for _ in utm]
-    upk = torch.zeros(2 * len(utm), dtype=torch.int32, device=dev)
-    uval = torch.zeros(len(utm), dtype=torch.float32, device=dev)
-    uft.SetBatch(ubufs, usizes, uouts, upk, uval)
-    uft.SetStream(cap)
-    g_u = capture(lambda: uft.FwdImageInvFFTBatch(spec, intra_op=conj), 1)
-    u_reps = max(2, K // 4)
-    u_ms = time_graph(g_u, u_reps, warm=3) / u_reps
-    uft.SetStream(torch.cuda.current_stream())
-    uneven = {
-        "templates": len(utm),
-        "sizes": "96 x 128^2, 96 x 256^2, 64 x 384^2 (circular masks), 512^2 window",
-        "value": round(len(utm) * 1000.0 / u_ms, 1),
-        "ms_per_call": round(u_ms, 5),
-        "form": uft.Describe("batch").splitlines()[0],
-        "timing": f"HIP graph of one FwdImageInvFFTBatch call replayed {u_reps} times",
-    }
-    del uouts, ubufs
+    if want("uneven"):
+        # 96 x 256^2, 64 x 384^2), one call per step
+        utm = W.batched_templates()
+        usizes = [(t.shape[1], t.shape[0]) for t in utm]
+        ubufs = [torch.from_numpy(W.to_padded(t).reshape(-1)).to(dev) for t in utm]
+        uft = plan()
+        uouts = [torch.zeros(uft.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev) for _ in utm]
+        upk = torch.zeros(2 * len(utm), dtype=torch.int32, device=dev)
+        uval = torch.zeros(len(utm), dtype=torch.float32, device=dev)
+        uft.SetBatch(ubufs, usizes, uouts, upk, uval)
+        uft.SetStream(cap)
+        g_u = capture(lambda: uft.FwdImageInvFFTBatch(spec, intra_op=conj), 1)
+        u_reps = max(2, K // 4)
+        u_ms = time_graph(g_u, u_reps, warm=3) / u_reps
+        uft.SetStream(torch.cuda.current_stream())
+        uneven = {
+            "templates": len(utm),
+            "sizes": "96 x 128^2, 96 x 256^2, 64 x 384^2 (circular masks), 512^2 window",
+            "value": round(len(utm) * 1000.0 / u_ms, 1),
+            "ms_per_call": round(u_ms, 5),
+            "form": uft.Describe("batch").splitlines()[0],
+            "timing": f"HIP graph of one FwdImageInvFFTBatch call replayed {u_reps} times",
+        }
+        del uouts, ubufs
 
     # ---------------------------------------------------------------- FP16 I/O: 256 FP16 templates of 512^2
-    # (radius-220 masks; 16 distinct, each listed 16 times) against the FP16 micrograph's __half2 spectrum,
-    # FP16 512^2 outputs; FP32 and split-BF16 correlation passes
-    T5, n5 = 512, 256
-    mft16 = F.FourierTransformer("f16", "c32h", 2)
-    mft16.SetForwardFFTPlan(N, N, 1, N, N, 1)
-    mft16.SetInverseFFTPlan(N, N, 1, N, N, 1)
-    spec16 = torch.from_numpy(W.to_padded(mic.astype(np.float16), dtype=np.float16).reshape(-1)).to(dev)
-    mft16.FwdFFT(spec16, pre_op=F.Scale(1.0 / N))
-    t5 = [torch.from_numpy(W.to_padded(W.masked_template(T5, 220.0, seed=500 + i).astype(np.float16),
-                                       dtype=np.float16).reshape(-1)).to(dev) for i in range(16)]
-    c5 = {}
-    for prec in ("fp32", "split_bf16"):
-        p5 = F.FourierTransformer("f16", "c32h", 2)
-        p5.SetForwardFFTPlan(T5, T5, 1, N, N, 1)
-        p5.SetInverseFFTPlan(N, N, 1, M, M, 1)
-        p5.SetOutputWindow(-M // 2, -M // 2)
-        o5 = [torch.zeros(p5.ReturnInvOutputMemorySize(), dtype=torch.float16, device=dev) for _ in range(n5)]
-        pk5 = torch.zeros(2 * n5, dtype=torch.int32, device=dev)
-        v5 = torch.zeros(n5, dtype=torch.float32, device=dev)
-        p5.SetPrecision(prec)
-        p5.SetBatch([t5[i % 16] for i in range(n5)], [(T5, T5)] * n5, o5, pk5, v5)
-        p5.SetStream(cap)
-        post5 = F.Scale(1.0 / (N * T5 * T5))
-        g5 = capture(lambda: p5.FwdImageInvFFTBatch(spec16, intra_op=conj, post_op=post5), 1)
-        r5 = max(2, K // 10)
-        ms5 = time_graph(g5, r5, warm=3) / r5
-        p5.SetStream(torch.cuda.current_stream())
-        c5[prec] = {"value": round(n5 * 1000.0 / ms5, 1), "ms_per_call": round(ms5, 5),
-                    "form": p5.Describe("batch").splitlines()[0], "peaks": pk5.cpu().numpy()}
-        del o5
-    # __half2 spectrum read once per call, each FP16 template in and each FP16 window out once
-    bytes5_call = 33570816 + n5 * (524288 + 524288)
-    config5 = {
-        "templates": n5,
-        "workload": "256 FP16 512^2 templates (radius-220 masks) vs the FP16 4096^2 micrograph's __half2 spectrum, "
-                    "FP16 512^2 window outputs",
-        "fp32": {k: v for k, v in c5["fp32"].items() if k != "peaks"},
-        "split_bf16": {k: v for k, v in c5["split_bf16"].items() if k != "peaks"},
-        "speedup_split_bf16_vs_fp32": round(c5["fp32"]["ms_per_call"] / c5["split_bf16"]["ms_per_call"], 3),
-        "peaks_identical": bool(np.array_equal(c5["fp32"]["peaks"], c5["split_bf16"]["peaks"])),
-        "compulsory_bytes_per_call": bytes5_call,
-        "hbm_frac_per_call_fp32": round(bytes5_call / (c5["fp32"]["ms_per_call"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "timing": "HIP graph of one FwdImageInvFFTBatch call replayed; parity in tests/test_gpu_mfma.py",
-    }
-    del t5, spec16
+    if want("config5"):
+        # (radius-220 masks; 16 distinct, each listed 16 times) against the FP16 micrograph's __half2 spectrum,
+        # FP16 512^2 outputs; FP32 and split-BF16 correlation passes
+        T5, n5 = 512, 256
+        mft16 = F.FourierTransformer("f16", "c32h", 2)
+        mft16.SetForwardFFTPlan(N, N, 1, N, N, 1)
+        mft16.SetInverseFFTPlan(N, N, 1, N, N, 1)
+        spec16 = torch.from_numpy(W.to_padded(mic.astype(np.float16), dtype=np.float16).reshape(-1)).to(dev)
+        mft16.FwdFFT(spec16, pre_op=F.Scale(1.0 / N))
+        t5 = [torch.from_numpy(W.to_padded(W.masked_template(T5, 220.0, seed=500 + i).astype(np.float16),
+                                           dtype=np.float16).reshape(-1)).to(dev) for i in range(16)]
+        c5 = {}
+        for prec in ("fp32", "split_bf16"):
+            p5 = F.FourierTransformer("f16", "c32h", 2)
+            p5.SetForwardFFTPlan(T5, T5, 1, N, N, 1)
+            p5.SetInverseFFTPlan(N, N, 1, M, M, 1)
+            p5.SetOutputWindow(-M // 2, -M // 2)
+            o5 = [torch.zeros(p5.ReturnInvOutputMemorySize(), dtype=torch.float16, device=dev) for _ in range(n5)]
+            pk5 = torch.zeros(2 * n5, dtype=torch.int32, device=dev)
+            v5 = torch.zeros(n5, dtype=torch.float32, device=dev)
+            p5.SetPrecision(prec)
+            p5.SetBatch([t5[i % 16] for i in range(n5)], [(T5, T5)] * n5, o5, pk5, v5)
+            p5.SetStream(cap)
+            post5 = F.Scale(1.0 / (N * T5 * T5))
+            g5 = capture(lambda: p5.FwdImageInvFFTBatch(spec16, intra_op=conj, post_op=post5), 1)
+            r5 = max(2, K // 10)
+            ms5 = time_graph(g5, r5, warm=3) / r5
+            p5.SetStream(torch.cuda.current_stream())
+            c5[prec] = {"value": round(n5 * 1000.0 / ms5, 1), "ms_per_call": round(ms5, 5),
+                        "form": p5.Describe("batch").splitlines()[0], "peaks": pk5.cpu().numpy()}
+            del o5
+        # __half2 spectrum read once per call, each FP16 template in and each FP16 window out once
+        bytes5_call = 33570816 + n5 * (524288 + 524288)
+        config5 = {
+            "templates": n5,
+            "workload": "256 FP16 512^2 templates (radius-220 masks) vs the FP16 4096^2 micrograph's __half2 spectrum, "
+                        "FP16 512^2 window outputs",
+            "fp32": {k: v for k, v in c5["fp32"].items() if k != "peaks"},
+            "split_bf16": {k: v for k, v in c5["split_bf16"].items() if k != "peaks"},
+            "speedup_split_bf16_vs_fp32": round(c5["fp32"]["ms_per_call"] / c5["split_bf16"]["ms_per_call"], 3),
+            "peaks_identical": bool(np.array_equal(c5["fp32"]["peaks"], c5["split_bf16"]["peaks"])),
+            "compulsory_bytes_per_call": bytes5_call,
+            "hbm_frac_per_call_fp32": round(bytes5_call / (c5["fp32"]["ms_per_call"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "timing": "HIP graph of one FwdImageInvFFTBatch call replayed; parity in tests/test_gpu_mfma.py",
+        }
+        del t5, spec16
 
     # ---------------------------------------------------------------- 3-D size change: padded forward
-    # 64^3 -> 256^3 and trimmed inverse 512^3 -> 128^3 (centred), each timed alone
-    def time_3d(setup):
-        p, fn = setup()
-        p.SetStream(cap)
-        with torch.cuda.stream(cap):
-            fn()  # first call: workspace and twiddle table, outside the capture
-        torch.cuda.synchronize()
-        g3 = capture(fn, 1)
-        r3 = max(2, K // 20)
-        ms3 = time_graph(g3, r3) / r3
-        p.SetStream(torch.cuda.current_stream())
-        return p, ms3
+    if want("config4"):
+        # 64^3 -> 256^3 and trimmed inverse 512^3 -> 128^3 (centred), each timed alone
+        def time_3d(setup):
+            p, fn = setup()
+            p.SetStream(cap)
+            with torch.cuda.stream(cap):
+                fn()  # first call: workspace and twiddle table, outside the capture
+            torch.cuda.synchronize()
+            g3 = capture(fn, 1)
+            r3 = max(2, K // 20)
+            ms3 = time_graph(g3, r3) / r3
+            p.SetStream(torch.cuda.current_stream())
+            return p, ms3
 
-    def fwd3():
-        p = F.FourierTransformer("f32", "c64", 3)
-        p.SetForwardFFTPlan(64, 64, 64, 256, 256, 256)
-        p.SetInverseFFTPlan(256, 256, 256, 256, 256, 256)
-        vin = torch.from_numpy(W.to_padded(np.random.default_rng(6).standard_normal((64, 64, 64)).astype(np.float32))
-                               .reshape(-1)).to(dev)
-        vout = torch.zeros(p.ReturnFwdOutputMemorySize(), dtype=torch.float32, device=dev)
-        return p, lambda: p.FwdFFT(vin, vout)
+        def fwd3():
+            p = F.FourierTransformer("f32", "c64", 3)
+            p.SetForwardFFTPlan(64, 64, 64, 256, 256, 256)
+            p.SetInverseFFTPlan(256, 256, 256, 256, 256, 256)
+            vin = torch.from_numpy(W.to_padded(np.random.default_rng(6).standard_normal((64, 64, 64)).astype(np.float32))
+                                   .reshape(-1)).to(dev)
+            vout = torch.zeros(p.ReturnFwdOutputMemorySize(), dtype=torch.float32, device=dev)
+            return p, lambda: p.FwdFFT(vin, vout)
 
-    def inv3():
-        p = F.FourierTransformer("f32", "c64", 3)
-        p.SetForwardFFTPlan(512, 512, 512, 512, 512, 512)
-        p.SetInverseFFTPlan(512, 512, 512, 128, 128, 128)
-        p.SetOutputWindow(-64, -64, -64)
-        sp3 = torch.randn(512 * 257 * 512 * 2, dtype=torch.float32, device=dev)
-        o3 = torch.zeros(p.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev)
-        return p, lambda: p.InvFFT(sp3, o3)
+        def inv3():
+            p = F.FourierTransformer("f32", "c64", 3)
+            p.SetForwardFFTPlan(512, 512, 512, 512, 512, 512)
+            p.SetInverseFFTPlan(512, 512, 512, 128, 128, 128)
+            p.SetOutputWindow(-64, -64, -64)
+            sp3 = torch.randn(512 * 257 * 512 * 2, dtype=torch.float32, device=dev)
+            o3 = torch.zeros(p.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev)
+            return p, lambda: p.InvFFT(sp3, o3)
 
-    _, f3_ms = time_3d(fwd3)
-    _, i3_ms = time_3d(inv3)
-    f3_bytes = 64 * 64 * 66 * 4 + 256 * 129 * 256 * 8
-    i3_bytes = 512 * 257 * 512 * 8 + 128 * 128 * 130 * 4
-    config4 = {
-        "fwd_64_to_256": {"ms": round(f3_ms, 5), "compulsory_bytes": f3_bytes,
-                          "hbm_frac": round(f3_bytes / (f3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "inv_512_to_128_centred": {"ms": round(i3_ms, 5), "compulsory_bytes": i3_bytes,
-                                   "hbm_frac": round(i3_bytes / (i3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "timing": "HIP graph of one transform, replayed; parity in tests/test_gpu_3d.py",
-    }
+        _, f3_ms = time_3d(fwd3)
+        _, i3_ms = time_3d(inv3)
+        f3_bytes = 64 * 64 * 66 * 4 + 256 * 129 * 256 * 8
+        i3_bytes = 512 * 257 * 512 * 8 + 128 * 128 * 130 * 4
+        config4 = {
+            "fwd_64_to_256": {"ms": round(f3_ms, 5), "compulsory_bytes": f3_bytes,
+                              "hbm_frac": round(f3_bytes / (f3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "inv_512_to_128_centred": {"ms": round(i3_ms, 5), "compulsory_bytes": i3_bytes,
+                                       "hbm_frac": round(i3_bytes / (i3_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "timing": "HIP graph of one transform, replayed; parity in tests/test_gpu_3d.py",
+        }
 
     # ---------------------------------------------------------------- single correlations
-    ft = plan()
-    stages = ft.Describe("xcorr").splitlines()[1:]
-    out = torch.zeros(ft.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev)
-    peak = torch.zeros(2, dtype=torch.int32, device=dev)
-    pval = torch.zeros(1, dtype=torch.float32, device=dev)
-    tbuf = torch.zeros(ft.ReturnInputMemorySize(), dtype=torch.float32, device=dev)
-    tbuf.copy_(tbufs[0])
+    if want("single"):
+        ft = plan()
+        stages = ft.Describe("xcorr").splitlines()[1:]
+        out = torch.zeros(ft.ReturnInvOutputMemorySize(), dtype=torch.float32, device=dev)
+        peak = torch.zeros(2, dtype=torch.int32, device=dev)
+        pval = torch.zeros(1, dtype=torch.float32, device=dev)
+        tbuf = torch.zeros(ft.ReturnInputMemorySize(), dtype=torch.float32, device=dev)
+        tbuf.copy_(tbufs[0])
 
-    def single_step():
-        ft.FwdImageInvFFTArgMax(tbuf, spec, out, peak, pval, intra_op=conj)
+        def single_step():
+            ft.FwdImageInvFFTArgMax(tbuf, spec, out, peak, pval, intra_op=conj)
 
-    for _ in range(args.warmup):
-        single_step()
-    torch.cuda.synchronize()
-    G = args.graph_batch
-    ft.SetStream(cap)
-    g_s = capture(single_step, G)
-    s_replays = max(1, K // G)
-    single_ms = time_graph(g_s, s_replays) / (s_replays * G)
-    stage_ms = {}
-    for i, name in enumerate(["r2c_x", "fused_y", "c2r_x"]):
-        gp = capture(lambda i=i: ft.RunStages("xcorr", i, i, tbuf, spec, out, intra_op=conj), G)
-        stage_ms[name] = time_graph(gp, s_replays) / (s_replays * G)
-    ft.SetStream(torch.cuda.current_stream())
-    w = N // 2 + 1
-    fused_bytes = 8 * w * (T + N + M)
+        for _ in range(args.warmup):
+            single_step()
+        torch.cuda.synchronize()
+        G = args.graph_batch
+        ft.SetStream(cap)
+        g_s = capture(single_step, G)
+        s_replays = max(1, K // G)
+        single_ms = time_graph(g_s, s_replays) / (s_replays * G)
+        stage_ms = {}
+        for i, name in enumerate(["r2c_x", "fused_y", "c2r_x"]):
+            gp = capture(lambda i=i: ft.RunStages("xcorr", i, i, tbuf, spec, out, intra_op=conj), G)
+            stage_ms[name] = time_graph(gp, s_replays) / (s_replays * G)
+        ft.SetStream(torch.cuda.current_stream())
+        fused_bytes = 8 * w * (T + N + M)
 
     if args.copy_calibration_mib:
         nb = args.copy_calibration_mib * (1 << 20) // 4
@@ -390,6 +402,7 @@ def main() -> int:
         torch.cuda.synchronize()
         del ca, cb
 
+    w = N // 2 + 1
     # compulsory bytes of one step counted once per batch: the image spectrum read once, each template
     # read once and each 512^2 window written once
     step_bytes = 8 * w * N + B * (4 * T * T + 4 * M * M)
@@ -459,7 +472,9 @@ def main() -> int:
             "peak_lag_template0": [b_pk[0] - M // 2, b_pk[1] - M // 2],
             "expected_peak_lag_template0": list(lags[0]),
         },
-        "split_bf16": {
+    }
+    if want("split_bf16"):
+        result["split_bf16"] = {
             "value": round(B * 1000.0 / s_ms, 1),
             "ms_per_step": round(s_ms, 5),
             "speedup_vs_fp32": round(ms_step / s_ms, 3),
@@ -474,12 +489,17 @@ def main() -> int:
                 "mfma_frac": round(s_mfma_tflops / 2500.0, 4),
                 "speedup_vs_fp32_pass": round(xc_ms / s_xc_ms, 3),
             },
-        },
-        "uneven_batch": uneven,
-        "stats_2dtm": stats_2dtm,
-        "config5_fp16": config5,
-        "config4_3d": config4,
-        "single": {
+        }
+    if want("uneven"):
+        result["uneven_batch"] = uneven
+    if want("stats"):
+        result["stats_2dtm"] = stats_2dtm
+    if want("config5"):
+        result["config5_fp16"] = config5
+    if want("config4"):
+        result["config4_3d"] = config4
+    if want("single"):
+        result["single"] = {
             "value": round(1000.0 / single_ms, 1),
             "ms_per_corr": round(single_ms, 5),
             "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
@@ -487,10 +507,9 @@ def main() -> int:
             "fused_y_algorithmic_bytes": fused_bytes,
             "timing": f"HIP graph of {G} fastfft_fwd_image_inv_argmax calls replayed {s_replays} times",
             "stages": stages,
-        },
-    }
+        }
 
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and want("cpu"):
         from oracle import cpu_baseline as CB
         from oracle import oracle as O
 
@@ -528,10 +547,12 @@ def main() -> int:
             "tolerance": 2e-6,
             "cpu_baseline_rel_l2_vs_fp64": float(f"{O.rel_l2(cb['window'].astype(np.float64), want):.3e}"),
         }
-        result["split_bf16"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(s_out0), want):.3e}")
-        result["split_bf16"]["tolerance"] = 2e-5
-        result["single"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(out.cpu().numpy()), want):.3e}")
-        result["single"]["argmax_match"] = [wx, wy] == [int(v) for v in peak.cpu().numpy()]
+        if want("split_bf16"):
+            result["split_bf16"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(s_out0), want):.3e}")
+            result["split_bf16"]["tolerance"] = 2e-5
+        if want("single"):
+            result["single"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(out.cpu().numpy()), want):.3e}")
+            result["single"]["argmax_match"] = [wx, wy] == [int(v) for v in peak.cpu().numpy()]
     print(json.dumps(result))
     return 0
 

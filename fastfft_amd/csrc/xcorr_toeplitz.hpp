@@ -9,31 +9,39 @@
 // a_t the column of template t's R2C x-pass output, the correlation's
 // second-axis pass is
 //   b_t[j] = sum_{v < L} S[j + v] conj(a_t[v]),          j < M,
-// which equals IFFT_y(FFT_y(S) conj(FFT_y(a_t)))[j] / Fy for j + v < Fy.  Over
-// the K = 2L interleaved values x_t = (Re a_t[0], Im a_t[0], Re a_t[1], ...):
-//   Re b_t[j] = sum_k R[2j + k] x_t[k],   R = (Sr[0], Si[0], Sr[1], Si[1], ...)
-//   Im b_t[j] = sum_k I[2j + k] x_t[k],   I = (Si[0], -Sr[0], Si[1], -Sr[1], ...)
-// two real matrix products with Toeplitz-structured left operands (rows j,
-// step 2) and the templates of a group as the 32 columns of the right one.
-// Each is taken on v_mfma_f32_32x32x16_bf16 as hi*hi + hi*lo + lo*hi of
-// split operands (hi = bf16(x), lo = bf16(x - hi)), accumulated in FP32.
+// which equals IFFT_y(FFT_y(S) conj(FFT_y(a_t)))[j] / Fy for j + v < Fy.  The
+// complex products are taken in Gauss's three-product form: with the real
+// sequences U0 = Sr + Si, U1 = Sr, U2 = Si and the template vectors
+// X0 = ar, X1 = -(ar + ai), X2 = ar - ai,
+//   k_p[j] = sum_v U_p[j + v] X_p[v],   Re b = k0 - k2,   Im b = k0 + k1,
+// three real matrix products with Toeplitz left operands (rows j) and the 32
+// templates of a group as the columns of the right one, K = L.  Each is taken on
+// v_mfma_f32_32x32x16_bf16 as hi*hi + hi*lo + lo*hi of split operands
+// (hi = bf16(x), lo = bf16(x - hi)), accumulated in FP32.
 //
 // The DC and Nyquist columns (real template values; the C2R pass keeps only
-// the real parts of their outputs) share one workgroup: R = (Re S_0[y], 0),
-// I = (0, Re S_last[y]) and x = (a_0[v], a_last[v]) give Re b_0 and Re b_last
-// as the two products.
+// the real parts of their outputs) share one workgroup: U0 = Re S_0,
+// U1 = Re S_last, X0 = a_0, X1 = a_last (U2 = X2 = 0) give them as k0 and k1.
 //
-// Workgroup (column pair index, template group of 32), four waves; wave w owns
-// output rows [128 w, 128 w + 128) as four 32-row tiles, for all 32 templates.
-// The left operands are staged in LDS once per workgroup as bf16 hi / lo
-// arrays in two copies shifted by 0 and 2 elements, so that every lane's
-// operand (rows step the Toeplitz diagonal by 2 elements) is two 8-byte
-// aligned reads; the copies start 64 bytes apart modulo 128, so the reads of
-// any sixteen lanes fall on distinct banks.  The template values are staged by
-// the workgroup in chunks of 64 rows (each thread its template's rows, split
-// into hi / lo on the way into LDS), double buffered: the loads of the next
-// chunk are in flight while the waves run the current one, and all four waves
-// read the same staged values.
+// Workgroup: one column (or the DC / Nyquist pair) and every template group of
+// the launch in turn; four waves, one per SIMD, wave w owns output rows
+// [128 w, 128 w + 128) as four 32-row tiles, for all 32 templates of a group.
+//   * Left operands: U_p split into hi / lo bf16, staged once per workgroup in
+//     LDS in four copies shifted by 0..3 elements, so that every lane's
+//     operand (8 consecutive elements from row j + 16 k + 8 g) is one 16-byte
+//     read at an 8-byte aligned index of copy j % 4; the copies start 32 bytes
+//     apart modulo 128, so the reads of sixteen lanes cover 128 distinct bytes.
+//   * Toeplitz reuse: tile a's operand at K-step k is tile 0's at K-step
+//     k + 2 a.  A chunk's even K-steps run first, then its odd ones; in each
+//     pass a wave keeps tile 0's operands of five K-steps of one parity in
+//     registers and reads one new operand per K-step and (p, hi / lo), one
+//     K-step ahead, after four at the start of the pass (7 reads per 4 K-steps
+//     where four tiles would take 16).
+//   * Right operands: the templates' X_p, split on the way into LDS, in chunks
+//     of eight K-steps, double buffered: the global loads of the next chunk are
+//     in flight while the waves run the current one.  The operands of the next
+//     K-step are read from LDS while the current K-step's 36 matrix
+//     instructions run.
 //
 // MFMA lane layouts (v_mfma_f32_32x32x16_bf16, checked on the device):
 //   A: lane l holds A[l % 32][8 (l / 32) + e], e < 8
@@ -51,239 +59,308 @@ typedef __attribute__((ext_vector_type(16))) float tz_f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned tz_u32x4;
 typedef __attribute__((ext_vector_type(2))) unsigned tz_u32x2;
 
-constexpr int kTzWaves     = 4;    // waves per workgroup
-constexpr int kTzRowsWave  = 128;  // output rows per wave (four 32-row tiles)
+constexpr int kTzWaves     = 4;                    // waves per workgroup (one per SIMD)
+constexpr int kTzTiles     = 4;                    // 32-row tiles per wave
+constexpr int kTzRowsWave  = 32 * kTzTiles;        // output rows per wave
 constexpr int kTzMaxM      = kTzWaves * kTzRowsWave;
-constexpr int kTzTemplates = 32;   // templates per workgroup (the MFMA's N)
-constexpr int kTzFy        = 1024; // frame length along y
-constexpr int kTzLen       = 2 * kTzFy;   // interleaved operand length
-constexpr int kTzCopy      = kTzLen + 32; // elements per shifted copy (copies 64 bytes apart modulo 128)
-constexpr int kTzArrays    = 4;           // R hi, R lo, I hi, I lo
-constexpr int kTzChunk     = 8;           // K-steps (8 template rows each) per staged chunk of templates
+constexpr int kTzTemplates = 32;                   // templates per group (the MFMA's N)
+constexpr int kTzFy        = 1024;                 // frame length along y
+constexpr int kTzMaxL      = 512;                  // template rows (K) at most
+constexpr int kTzCopyLen   = kTzFy + 16;           // elements per shifted copy (copies 32 bytes apart modulo 128)
+constexpr int kTzChunk     = 8;                    // K-steps (16 template rows each) per staged chunk
 
-__device__ __forceinline__ tz_f32x16 tz_mfma(tz_bf16x8 a, tz_bf16x8 b, tz_f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ tz_f32x16 tz_mfma(tz_u32x4 a, tz_u32x4 b, tz_f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(tz_bf16x8, a), __builtin_bit_cast(tz_bf16x8, b), c, 0,
+                                                  0, 0);
 }
 
-// hi = bf16(x) (round to nearest even), lo = bf16(x - hi) of two values, packed
-// as two bf16 each (element 0 in the low half)
-struct TzPair {
-    unsigned hi, lo;
-};
-__device__ __forceinline__ TzPair tz_split2(float x0, float x1) {
-    const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
-    const __bf16 l0 = (__bf16)(x0 - (float)h0), l1 = (__bf16)(x1 - (float)h1);
-    TzPair       p;
-    p.hi = (unsigned)__builtin_bit_cast(unsigned short, h0) | ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
-    p.lo = (unsigned)__builtin_bit_cast(unsigned short, l0) | ((unsigned)__builtin_bit_cast(unsigned short, l1) << 16);
-    return p;
+// hi = bf16(x) (round to nearest even) and lo = bf16(x - hi), as bf16 bit patterns
+__device__ __forceinline__ void tz_split(float x, unsigned& hi, unsigned& lo) {
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
+    hi             = __builtin_bit_cast(unsigned short, h);
+    lo             = __builtin_bit_cast(unsigned short, l);
 }
 
-// 16 bytes of LDS at an 8-byte aligned element offset, as two 8-byte reads
-__device__ __forceinline__ tz_bf16x8 tz_read8(const unsigned short* p) {
-    const tz_u32x2 a = *reinterpret_cast<const tz_u32x2*>(p);
-    const tz_u32x2 b = *reinterpret_cast<const tz_u32x2*>(p + 4);
-    const tz_u32x4 v = {a[0], a[1], b[0], b[1]};
-    return __builtin_bit_cast(tz_bf16x8, v);
-}
-
-// Body of k_xcorr_toeplitz for ordinary columns (ENDS = false) or the DC / Nyquist pair.
-// The workgroup's LDS:
-//   op  left operands: R and I, hi and lo, copies shifted by 0 and 2 elements
-//   tp  right operands of a chunk of K-steps, double buffered: [buffer][K-step][lane half
-//       g][template n] 8 bf16 (rows 8 ks + 4 g .. + 3, re and im interleaved), hi then lo
+// Workgroup LDS:
+//   u  left operands: u[p][h][c][i] = U_p^h[i + c] (h = 0 hi, 1 lo; c the copy's shift)
+//   x  right operands of a chunk of K-steps, double buffered: x[buf][k][p][h][g][n] holds
+//      X_p^h rows 16 k + 8 g .. + 7 of template n (8 bf16)
 struct TzShared {
-    __attribute__((aligned(16))) unsigned short op[kTzArrays][2][kTzCopy];
-    tz_u32x4                                    tp[2][kTzChunk][2][kTzTemplates][2];
+    __attribute__((aligned(16))) unsigned short u[3][2][4][kTzCopyLen];
+    tz_u32x4                                    x[2][kTzChunk][3][2][2][kTzTemplates];
 };
 
 template <bool ENDS>
 __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restrict__ items, int nitems, long a_es,
                                         const float2* __restrict__ S, long s_stride, long o_es, int M, int Lmax,
                                         float scale, int last, int col) {
-    auto&     s_op = sh.op;
-    auto&     s_tp = sh.tp;
-    const int tid  = threadIdx.x;
-    const int lane = tid & 63;
-    const int w    = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int g    = lane >> 5; // lane half: K rows 8 g .. 8 g + 7 of a K-step
-    const int n    = lane & 31; // MFMA row (A) / column (B)
-    const int nks  = (Lmax + 7) / 8;
-    const int nchk = (nks + kTzChunk - 1) / kTzChunk;
+    const int tid     = threadIdx.x;
+    const int lane    = tid & 63;
+    const int w       = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g       = lane >> 5; // lane half: K rows 8 g .. 8 g + 7 of a K-step
+    const int n       = lane & 31; // MFMA row (A) / column (B)
+    const int nks     = (Lmax + 15) / 16;
+    const int nchk    = (nks + kTzChunk - 1) / kTzChunk;
+    const int ngroups = (nitems + kTzTemplates - 1) / kTzTemplates;
+    const int nq      = ngroups * nchk; // staged chunks over all groups
 
-    // ---- staging of the templates: thread tid takes template tn = tid % 32 and the two
-    // (K-step, half) slots q = tid / 32 and q + 8 of a chunk, four rows each.  Every load
-    // is issued (rows clamped to the template, threads past the batch on template 0) and
-    // the values outside the template replaced by zeros, so no load waits for another.
-    const int       tn = tid & 31;
-    const int       tq = tid >> 5;
-    const int       tt = (int)blockIdx.y * kTzTemplates + tn;
-    const XcorrItem it = items[tt < nitems ? tt : 0];
-    const int       Lt = tt < nitems ? it.L : 0; // rows taken
-    const int       Lc = it.L > 0 ? it.L : 1;    // rows that may be read
-    const auto*     ga = as_global(static_cast<const float2*>(it.a));
-    auto load_chunk = [&](int ck, float2 (&z)[2][4]) {
+    // ---- staging of the right operands: thread tid takes template tn = tid % 32 of the
+    // group and K-step tk = tid / 32 of the chunk (rows 16 tk .. 16 tk + 15).  Every load is
+    // issued (rows clamped to the template, threads past the batch on template 0) and the
+    // values outside the template replaced by zeros, so no load waits for another.
+    const int tn = tid & 31;
+    const int tk = tid >> 5;
+    float2    z[16];
+    auto      load_chunk = [&](int q) {
+        const int       grp = q / nchk, ck = q % nchk;
+        const int       tt  = grp * kTzTemplates + tn;
+        const XcorrItem it  = items[tt < nitems ? tt : 0];
+        const int       Lc  = it.L > 0 ? it.L : 1;
+        const auto*     ga  = as_global(static_cast<const float2*>(it.a));
+        const int       v0  = 16 * (ck * kTzChunk + tk);
+        // element offsets in 32 bits (a template column spans at most 512 rows of a_es
+        // elements), computed per load so that none is kept across the chunk
+        int es = (int)a_es;
+        asm volatile("" : "+s"(es));
 #pragma unroll
-        for ( int sl = 0; sl < 2; sl++ ) {
-            const int q  = tq + 8 * sl; // 0 .. 15: K-step q / 2, half q % 2
-            const int v0 = 8 * (ck * kTzChunk + (q >> 1)) + 4 * (q & 1);
+        for ( int i = 0; i < 16; i++ ) {
+            const int r = min(v0 + i, Lc - 1) * es;
+            if constexpr ( ENDS )
+                z[i] = make_float2(load_cpx(ga, r).x, load_cpx(ga, r + last).x);
+            else
+                z[i] = load_cpx(ga, r + col);
+        }
+    };
+    auto store_chunk = [&](int q, int into) {
+        const int grp = q / nchk, ck = q % nchk;
+        const int tt  = grp * kTzTemplates + tn;
+        const int Lt  = tt < nitems ? items[tt].L : 0; // rows taken
+        const int v0  = 16 * (ck * kTzChunk + tk);
+        auto&     dst = sh.x[into][tk];
 #pragma unroll
-            for ( int i = 0; i < 4; i++ ) {
-                const long r = (long)min(v0 + i, Lc - 1) * a_es;
-                if constexpr ( ENDS )
-                    z[sl][i] = make_float2(load_cpx(ga, r).x, load_cpx(ga, r + last).x);
-                else
-                    z[sl][i] = load_cpx(ga, r + col);
+        for ( int gg = 0; gg < 2; gg++ ) {
+            unsigned hw[3][8], lw[3][8];
+#pragma unroll
+            for ( int i = 0; i < 8; i++ ) {
+                const bool   in = v0 + 8 * gg + i < Lt;
+                const float2 a  = in ? z[8 * gg + i] : make_float2(0.f, 0.f);
+                float        x0, x1, x2;
+                if constexpr ( ENDS ) {
+                    x0 = a.x, x1 = a.y, x2 = 0.f;
+                }
+                else {
+                    x0 = a.x, x1 = -(a.x + a.y), x2 = a.x - a.y;
+                }
+                tz_split(x0, hw[0][i], lw[0][i]);
+                tz_split(x1, hw[1][i], lw[1][i]);
+                tz_split(x2, hw[2][i], lw[2][i]);
+            }
+#pragma unroll
+            for ( int p = 0; p < 3; p++ ) {
+                tz_u32x4 h, l;
+#pragma unroll
+                for ( int i = 0; i < 4; i++ ) {
+                    h[i] = hw[p][2 * i] | (hw[p][2 * i + 1] << 16);
+                    l[i] = lw[p][2 * i] | (lw[p][2 * i + 1] << 16);
+                }
+                dst[p][0][gg][tn] = h;
+                dst[p][1][gg][tn] = l;
             }
         }
     };
-    auto store_chunk = [&](int buf, int ck, const float2 (&z)[2][4]) {
-#pragma unroll
-        for ( int sl = 0; sl < 2; sl++ ) {
-            const int q  = tq + 8 * sl;
-            const int v0 = 8 * (ck * kTzChunk + (q >> 1)) + 4 * (q & 1);
-            tz_u32x4  h, l;
-#pragma unroll
-            for ( int i = 0; i < 4; i++ ) {
-                const bool   in = v0 + i < Lt;
-                const TzPair p  = tz_split2(in ? z[sl][i].x : 0.f, in ? z[sl][i].y : 0.f);
-                h[i]            = p.hi;
-                l[i]            = p.lo;
-            }
-            s_tp[buf][q >> 1][q & 1][tn][0] = h;
-            s_tp[buf][q >> 1][q & 1][tn][1] = l;
-        }
-    };
-    float2 zc[2][4];
-    load_chunk(0, zc);
+    load_chunk(0);
 
-    // ---- left operands
+    // ---- left operands: thread tid writes indices 4 tid .. 4 tid + 3 of every copy, from
+    // U_p[4 tid .. 4 tid + 6] (zero past the frame)
     {
         const auto* s0 = as_global(S) + (long)col * s_stride;
         const auto* s1 = as_global(S) + (long)last * s_stride;
+        const int   y0 = 4 * tid;
+        unsigned    uh[3][7], ul[3][7];
 #pragma unroll
-        for ( int yy = 0; yy < kTzFy / (kTzWaves * 64); yy++ ) {
-            const int y = tid + kTzWaves * 64 * yy;
-            float     r0, r1, i0, i1; // R[2y], R[2y + 1], I[2y], I[2y + 1]
-            if constexpr ( ! ENDS ) {
-                const float2 v = load_cpx(s0, y);
-                r0 = v.x, r1 = v.y, i0 = v.y, i1 = -v.x;
-            }
-            else {
-                r0 = load_cpx(s0, y).x, r1 = 0.f, i0 = 0.f, i1 = load_cpx(s1, y).x;
-            }
-            const TzPair pr = tz_split2(r0, r1), pi = tz_split2(i0, i1);
-#pragma unroll
-            for ( int c = 0; c < 2; c++ ) {
-                const int at = 2 * y - 2 * c; // copy c holds element at + 2 c at index at
-                if ( at >= 0 ) {
-                    *reinterpret_cast<unsigned*>(&s_op[0][c][at]) = pr.hi;
-                    *reinterpret_cast<unsigned*>(&s_op[1][c][at]) = pr.lo;
-                    *reinterpret_cast<unsigned*>(&s_op[2][c][at]) = pi.hi;
-                    *reinterpret_cast<unsigned*>(&s_op[3][c][at]) = pi.lo;
+        for ( int i = 0; i < 7; i++ ) {
+            const int y = y0 + i;
+            float     u0 = 0.f, u1 = 0.f, u2 = 0.f;
+            if ( y < kTzFy ) {
+                if constexpr ( ENDS ) {
+                    u0 = load_cpx(s0, y).x;
+                    u1 = load_cpx(s1, y).x;
+                }
+                else {
+                    const float2 v = load_cpx(s0, y);
+                    u0 = v.x + v.y, u1 = v.x, u2 = v.y;
                 }
             }
+            tz_split(u0, uh[0][i], ul[0][i]);
+            tz_split(u1, uh[1][i], ul[1][i]);
+            tz_split(u2, uh[2][i], ul[2][i]);
         }
+#pragma unroll
+        for ( int p = 0; p < 3; p++ )
+#pragma unroll
+            for ( int c = 0; c < 4; c++ ) {
+                const tz_u32x2 h = {uh[p][c] | (uh[p][c + 1] << 16), uh[p][c + 2] | (uh[p][c + 3] << 16)};
+                const tz_u32x2 l = {ul[p][c] | (ul[p][c + 1] << 16), ul[p][c + 2] | (ul[p][c + 3] << 16)};
+                *reinterpret_cast<tz_u32x2*>(&sh.u[p][0][c][y0]) = h;
+                *reinterpret_cast<tz_u32x2*>(&sh.u[p][1][c][y0]) = l;
+            }
     }
-    store_chunk(0, 0, zc);
+    store_chunk(0, 0);
     __syncthreads( );
 
-    // rows of this wave: all four tiles are computed (rows < 512 read valid operands),
-    // tiles at or past M are not stored
-    const int row0 = w * kTzRowsWave;
-    tz_f32x16 acc_r[4], acc_i[4];
+    // lane's left operand of tile 0 at K-step k, part (p, h): copy c = n % 4 at index
+    // 128 w + n - c + 8 g + 16 k (a multiple of 4)
+    const int                    cpy  = n & 3;
+    const unsigned short* const  ubase = &sh.u[0][0][cpy][kTzRowsWave * w + n - cpy + 8 * g];
+    constexpr int                kPart = 4 * kTzCopyLen; // elements between (p, h) parts
+    auto read_a = [&](int k, int p, int h) -> tz_u32x4 {
+        const unsigned short* e = ubase + kPart * (2 * p + h) + 16 * k;
+        const tz_u32x2        a = *reinterpret_cast<const tz_u32x2*>(e);
+        const tz_u32x2        b = *reinterpret_cast<const tz_u32x2*>(e + 4);
+        return tz_u32x4{a[0], a[1], b[0], b[1]};
+    };
+    // A chunk's K-steps run even ones first, then odd ones (the sum does not depend on the
+    // order), so that only one parity's operands are live: within a parity pass, K-step
+    // ks = 8 ck + par + 2 i uses slot (i + a) % 5 for tile a, which holds tile 0's operand of
+    // K-step ks + 2 a.  A pass reads four operands ahead, then during K-step i the one that
+    // tile 3 takes at K-step i + 1 (into the slot tile 0 of K-step i - 1 left).
+    tz_u32x4  aslot[5][3][2];
+    tz_u32x4  bop[2][3][2]; // right operands of the current and the next K-step
+    tz_f32x16 acc[kTzTiles][3];
+
+    for ( int grp = 0; grp < ngroups; grp++ ) {
 #pragma unroll
-    for ( int a = 0; a < 4; a++ ) {
-        acc_r[a] = tz_f32x16{ };
-        acc_i[a] = tz_f32x16{ };
-    }
-    // LDS element of this lane's left operand: lane row j = row0 + 32 a + n, operand offset
-    // 2 j + 16 ks + 8 g, in copy c = j % 2 (= n % 2) at index offset - 2 c (a multiple of 4)
-    const int cpy  = n & 1;
-    const int base = 2 * (row0 + n) + 8 * g - 2 * cpy;
-    for ( int ck = 0; ck < nchk; ck++ ) {
-        const int buf = ck & 1;
-        if ( ck + 1 < nchk )
-            load_chunk(ck + 1, zc);
-        const int kend = min(kTzChunk, nks - ck * kTzChunk);
-        for ( int kk = 0; kk < kend; kk++ ) {
-            const int       ks = ck * kTzChunk + kk;
-            const tz_bf16x8 bh = __builtin_bit_cast(tz_bf16x8, s_tp[buf][kk][g][n][0]);
-            const tz_bf16x8 bl = __builtin_bit_cast(tz_bf16x8, s_tp[buf][kk][g][n][1]);
-            // left operands of the four tiles, then the products in three rounds over the
-            // eight accumulators, so that dependent matrix instructions are eight apart
-            tz_bf16x8 rh[4], rl[4], ih[4], il[4];
+      for ( int a = 0; a < kTzTiles; a++ )
 #pragma unroll
-            for ( int a = 0; a < 4; a++ ) {
-                const int at = base + 64 * a + 16 * ks;
-                rh[a]        = tz_read8(&s_op[0][cpy][at]);
-                rl[a]        = tz_read8(&s_op[1][cpy][at]);
-                ih[a]        = tz_read8(&s_op[2][cpy][at]);
-                il[a]        = tz_read8(&s_op[3][cpy][at]);
-            }
+        for ( int p = 0; p < 3; p++ )
+            acc[a][p] = tz_f32x16{ };
+      for ( int ck = 0; ck < nchk; ck++ ) {
+        const int q   = grp * nchk + ck;
+        const int buf = q & 1;
+        // the next chunk's templates (the last chunk stages itself again into the idle
+        // buffer, so that no branch depends on the chunk count)
+        const int qn = min(q + 1, nq - 1);
+        load_chunk(qn);
 #pragma unroll
-            for ( int a = 0; a < 4; a++ ) {
-                acc_r[a] = tz_mfma(rh[a], bh, acc_r[a]);
-                acc_i[a] = tz_mfma(ih[a], bh, acc_i[a]);
-            }
+        for ( int par = 0; par < 2; par++ ) {
+            // every K-step of a chunk runs: those past nks meet zero template rows (the staging
+            // zero-fills rows past each template), and their left operands are read at K-steps
+            // clamped to nks + 5, inside the staged column, so no branch splits the pass
+            const int k0 = ck * kTzChunk + par; // first K-step of the pass
+            const int kc = nks + 5;             // largest K-step whose left operand is read
 #pragma unroll
-            for ( int a = 0; a < 4; a++ ) {
-                acc_r[a] = tz_mfma(rh[a], bl, acc_r[a]);
-                acc_i[a] = tz_mfma(ih[a], bl, acc_i[a]);
-            }
+            for ( int sl = 0; sl < 4; sl++ )
 #pragma unroll
-            for ( int a = 0; a < 4; a++ ) {
-                acc_r[a] = tz_mfma(rl[a], bh, acc_r[a]);
-                acc_i[a] = tz_mfma(il[a], bh, acc_i[a]);
+                for ( int p = 0; p < 3; p++ )
+#pragma unroll
+                    for ( int h = 0; h < 2; h++ )
+                        aslot[sl][p][h] = read_a(min(k0 + 2 * sl, kc), p, h);
+#pragma unroll
+            for ( int p = 0; p < 3; p++ )
+#pragma unroll
+                for ( int h = 0; h < 2; h++ )
+                    bop[0][p][h] = sh.x[buf][par][p][h][g][n];
+#pragma unroll
+            for ( int i = 0; i < kTzChunk / 2; i++ ) {
+                const int ks = k0 + 2 * i;
+                // the next K-step's operands: tile 3's (tile 0's of K-step ks + 8) and the right ones
+                if ( i + 1 < kTzChunk / 2 ) {
+#pragma unroll
+                    for ( int p = 0; p < 3; p++ )
+#pragma unroll
+                        for ( int h = 0; h < 2; h++ )
+                            aslot[(i + 4) % 5][p][h] = read_a(min(ks + 8, kc), p, h);
+#pragma unroll
+                    for ( int p = 0; p < 3; p++ )
+#pragma unroll
+                        for ( int h = 0; h < 2; h++ )
+                            bop[(i + 1) & 1][p][h] = sh.x[buf][par + 2 * (i + 1)][p][h][g][n];
+                }
+                // hi*hi, hi*lo, lo*hi over the twelve accumulators
+#pragma unroll
+                for ( int t = 0; t < 3; t++ ) {
+                    const int ha = t == 2 ? 1 : 0, hb = t == 1 ? 1 : 0;
+#pragma unroll
+                    for ( int p = 0; p < 3; p++ )
+#pragma unroll
+                        for ( int a = 0; a < kTzTiles; a++ )
+                            acc[a][p] = tz_mfma(aslot[(i + a) % 5][p][ha], bop[i & 1][p][hb], acc[a][p]);
+                }
+                // issue order of the K-step: the next K-step's twelve LDS reads two at a time
+                // between the first six matrix instructions (read a full K-step ahead, not
+                // pushed back to where the current operands die), then the other thirty
+                if ( i + 1 < kTzChunk / 2 ) {
+#pragma unroll
+                    for ( int j = 0; j < 6; j++ ) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); // one MFMA
+                        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // two LDS reads
+                    }
+                    __builtin_amdgcn_sched_group_barrier(0x008, 30, 0);
+                }
+                // K-steps do not interleave (that would hold more operands live)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if ( ck + 1 < nchk ) {
-            // buffer buf ^ 1 was last read in chunk ck - 1, before the barrier that ended it
-            store_chunk(buf ^ 1, ck + 1, zc);
+        // the next chunk's templates into the other buffer (last read in chunk q - 1, before
+        // the barrier that ended it): their loads had the whole chunk's matrix work to arrive
+        store_chunk(qn, buf ^ 1);
+        if ( q + 1 < nq )
             __syncthreads( );
-        }
-    }
-    // ---- store: register r of tile a holds row row0 + 32 a + 8 (r / 4) + 4 g + r % 4,
-    // template n of the group
-    const int tout = (int)blockIdx.y * kTzTemplates + n;
-    if ( tout >= nitems )
-        return;
-    auto* go = as_global(static_cast<float2*>(items[tout].out));
+      }
+      {
+            // ---- the group's outputs: register r of tile a holds row
+            // 128 w + 32 a + 8 (r / 4) + 4 g + r % 4 of template n of the group
+            const int tout = grp * kTzTemplates + n;
+            if ( tout < nitems ) {
+                // 32-bit element offsets: the row steps are wave-uniform multiples of o_es,
+                // formed here (the empty asm keeps the compiler from hoisting 64 of them out
+                // of the chunk loop, where they would not fit in registers)
+                int oes = (int)o_es, mm = M;
+                asm volatile("" : "+s"(oes), "+s"(mm));
+                auto*     go  = as_global(static_cast<float2*>(items[tout].out));
 #pragma unroll
-    for ( int a = 0; a < 4; a++ ) {
-        if ( row0 + 32 * a < M ) {
+                for ( int a = 0; a < kTzTiles; a++ ) {
+                    const int rowa = kTzRowsWave * w + 32 * a;
+                    if ( rowa < mm ) {
+                        auto* rp = go + (rowa + 4 * g) * oes;
 #pragma unroll
-            for ( int r = 0; r < 16; r++ ) {
-                const int j = row0 + 32 * a + 8 * (r >> 2) + 4 * g + (r & 3);
-                if ( j < M ) {
-                    if constexpr ( ! ENDS ) {
-                        store_cpx(go, (long)j * o_es + col, make_float2(acc_r[a][r] * scale, acc_i[a][r] * scale));
+                        for ( int r = 0; r < 16; r++ ) {
+                            const int dj = 8 * (r >> 2) + (r & 3);
+                            if ( rowa + 4 * g + dj < mm ) {
+                                const float k0 = acc[a][0][r], k1 = acc[a][1][r], k2 = acc[a][2][r];
+                                if constexpr ( ! ENDS ) {
+                                    store_cpx(rp, dj * oes + col, make_float2((k0 - k2) * scale, (k0 + k1) * scale));
+                                }
+                                else {
+                                    store_cpx(rp, dj * oes, make_float2(k0 * scale, 0.f));
+                                    store_cpx(rp, dj * oes + last, make_float2(k1 * scale, 0.f));
+                                }
+                            }
+                        }
                     }
-                    else {
-                        store_cpx(go, (long)j * o_es, make_float2(acc_r[a][r] * scale, 0.f));
-                        store_cpx(go, (long)j * o_es + last, make_float2(acc_i[a][r] * scale, 0.f));
-                    }
+                    // one tile's accumulators at a time out of the matrix registers
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
-        }
+      }
     }
 }
 
-// Grid: x = last workgroups when pack_ends (columns 1 .. last - 1, and workgroup 0
-// for the pair 0 / last), else last + 1; y = ceil(nitems / 32) template groups;
-// 256 threads.
+// Grid: x = last workgroups (columns 1 .. last - 1, and workgroup 0 for the pair
+// 0 / last); 256 threads; every workgroup runs all ceil(nitems / 32) template groups.
 //   S       frame image in mixed form, column c at S + c * s_stride (Fy = 1024 values)
 //   items   the templates: a[v * a_es + c] element v of column c of template t
-//           (1 <= L_t rows), out[j * o_es + c] element j of its output column
-//   M       output rows (<= 512), Lmax the largest template's rows (L_t <= Lmax,
-//           Lmax rounded up to 8 <= 512), last the Nyquist column index (ncols - 1)
-__global__ void __launch_bounds__(kTzWaves * 64, 2)
+//           (1 <= L_t <= Lmax rows), out[j * o_es + c] element j of its output column
+//   M       output rows (<= 512), Lmax the largest template's rows (<= 512, with
+//           Lmax + M - 1 <= 1024), last the Nyquist column index (ncols - 1)
+__global__ void __launch_bounds__(kTzWaves * 64, 1)
         k_xcorr_toeplitz(const XcorrItem* __restrict__ items, int nitems, long a_es, const float2* __restrict__ S,
-                         long s_stride, long o_es, int M, int Lmax, float scale, int last, int pack_ends) {
+                         long s_stride, long o_es, int M, int Lmax, float scale, int last) {
     __shared__ TzShared sh;
     // each XCD takes a contiguous range of columns (template and output lines shared in one L2)
     const int col = __builtin_amdgcn_readfirstlane(xcd_row(blockIdx.x, gridDim.x));
-    if ( pack_ends && col == 0 ) // workgroup 0 takes columns 0 and last
+    if ( col == 0 ) // workgroup 0 takes columns 0 and last
         tz_body<true>(sh, items, nitems, a_es, S, s_stride, o_es, M, Lmax, scale, last, col);
     else
         tz_body<false>(sh, items, nitems, a_es, S, s_stride, o_es, M, Lmax, scale, last, col);
