@@ -451,8 +451,15 @@ static size_t map_extent(const HostLineMap& m, int nlines, int nelem) {
 // A slot: the R2C rows' output (nlines template rows of nout values); B slot: the
 // correlation pass's output as the C2R pass reads it (nlines rows of the spectrum's
 // w values), rounded to even counts so that every slot starts 16-byte aligned.
-static size_t slot_elems_a(const Stage& r2c) { return (map_extent(r2c.om, r2c.nlines, r2c.nout) + 1) / 2 * 2; }
-static size_t slot_elems_b(const Stage& c2r) { return (map_extent(c2r.im, c2r.nlines, c2r.N / 2 + 1) + 1) / 2 * 2; }
+// Column-major slots (s1 = 1, es = the column length) hold whole columns: the row kernels
+// move groups of rows past the last row, and the split-BF16 pass reads 16-row groups.
+static size_t slot_extent(const HostLineMap& m, int nlines, int nelem) {
+    if ( m.s1 == 1 && m.es > 1 )
+        return (size_t)nelem * (size_t)m.es;
+    return map_extent(m, nlines, nelem);
+}
+static size_t slot_elems_a(const Stage& r2c) { return (slot_extent(r2c.om, r2c.nlines, r2c.nout) + 1) / 2 * 2; }
+static size_t slot_elems_b(const Stage& c2r) { return (slot_extent(c2r.im, c2r.nlines, c2r.N / 2 + 1) + 1) / 2 * 2; }
 
 // Device block for chunks of C templates: A slots, B slots, argmax keys, then the
 // descriptor arrays of all n templates (filled by fill_batch_items).
@@ -531,6 +538,7 @@ static fastfft_status run_batch_chunks(fastfft_plan p, const std::vector<Stage>&
             io.scale   = intra;
             io.xitems     = p->batch_xc + c0;
             io.nxitems    = cnt;
+            io.xout_step  = (long)slot_elems_b(st[2]);
             io.xgroup     = p->batch_group;
             io.split_bf16 = p->precision == FASTFFT_PRECISION_SPLIT_BF16;
             io.img_mixed  = mixed;
@@ -783,6 +791,21 @@ static PlanState frame_plan(int input_type, const Dims& n, const Dims& out, cons
     return ps;
 }
 
+// Column-major slots of a frame batch's stage list (R2C rows, correlation, C2R rows):
+// template spectra A[kx][v] (columns of la >= template rows, a multiple of 16) and
+// correlation outputs B[kx][j] (columns of mb >= window rows, a multiple of 8), so
+// that the correlation passes read and write whole columns; the row kernels move
+// their rows through LDS tiles (rows_packed.hpp).
+static void column_major_slots(std::vector<Stage>& xc) {
+    Stage &    r = xc[0], &x = xc[1], &c = xc[2];
+    const long la = (r.nlines + 15) / 16 * 16;
+    const long mb = (c.nlines + 7) / 8 * 8;
+    r.om          = HostLineMap{1, 0, la, 1};
+    x.im          = HostLineMap{la, 0, 1, 1};
+    x.om          = HostLineMap{mb, 0, 1, 1};
+    c.im          = HostLineMap{1, 0, mb, 1};
+}
+
 // Activates the sub-frame form of the batch (see fastfft_plan_s) when the frame
 // is at most a quarter of the plan's, FASTFFT_NO_SUBFRAME is unset and the
 // batched kernels cover the frame's correlation stages; otherwise leaves it off.
@@ -810,6 +833,14 @@ static fastfft_status setup_subframe(fastfft_plan p) {
     std::vector<Stage> xc, inv, fwd;
     if ( validate(ps).code || build_stages(ps, "xcorr", xc).code || xc.size( ) != 3 )
         return ok( );
+    // column-major template and output slots, unless FASTFFT_BATCH_ROW_MAJOR=1 (or the
+    // column-major row kernels do not cover the frame)
+    if ( ! env_flag("FASTFFT_BATCH_ROW_MAJOR") ) {
+        std::vector<Stage> cm = xc;
+        column_major_slots(cm);
+        if ( batch_kernels_cover(cm[0], cm[1], cm[2], CpxT::c64) )
+            xc = std::move(cm);
+    }
     if ( ! batch_kernels_cover(xc[0], xc[1], xc[2], CpxT::c64) )
         return ok( );
     // the image reduced to the frame: trimmed inverse of the N-frame spectrum (read in the plan's image type) ...

@@ -39,6 +39,9 @@ struct StageIO {
     int         nxitems = 0;
     // xcorr with xitems: templates per template group along gridDim.y (0: the default)
     int xgroup = 0;
+    // xcorr with xitems: elements between the outputs of consecutive items (the batch's B
+    // slots are equally spaced), or 0 when they are not
+    long xout_step = 0;
     // xcorr only: run the split-BF16 matrix-core kernel where it covers the stage
     // (xcorr_toeplitz_covers, with img_mixed), the FP32 kernels elsewhere
     bool split_bf16 = false;

@@ -72,10 +72,27 @@ static bool r2c_rows_shape(const Stage& s) {
            s.P * s.Q <= kTwiddleN;
 }
 
+// Column-major batch output: row r at r, element k at k * es (es even, every column
+// holding whole groups of kCmRows rows).
+static bool r2c_cm_output(const Stage& s) {
+    return s.om.n2 == 1 && s.om.s1 == 1 && s.om.es > 1 && s.om.es % 2 == 0 &&
+           s.om.es >= (s.nlines + kCmRows - 1) / kCmRows * kCmRows;
+}
+
 bool r2c_rows_batch_covers(const Stage& s) {
+    if ( r2c_cm_output(s) )
+        return r2c_packed_covers(s) && s.im.n2 == 1 && s.im.es == 1;
     // one template per grid row: single-index lines, unit-stride row-major output with even leading dimension
     return (r2c_rows_shape(s) || r2c_packed_covers(s)) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1 && s.om.es == 1 &&
            s.om.s1 % 2 == 0;
+}
+
+template <int P, typename TIn>
+static hipError_t go_r2c_packed_cm(const Stage& s, const RowBatchItem* items, int n, int max_rows, float pre,
+                                   const float2* tw, hipStream_t st) {
+    const dim3 grid((unsigned)((max_rows + kCmRows - 1) / kCmRows), (unsigned)n);
+    k_r2c_rows_packed_cm<P, TIn><<<grid, dim3(64 * kPackedRows), 0, st>>>(s.om.es, s.nout, pre, tw, items);
+    return hipGetLastError( );
 }
 
 hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,
@@ -83,6 +100,15 @@ hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int m
     if ( ! r2c_rows_batch_covers(s) )
         return hipErrorNotSupported;
     const auto* it = static_cast<const RowBatchItem*>(items);
+    if ( r2c_cm_output(s) ) {
+        if ( max_rows < 1 || n < 1 )
+            return hipErrorNotSupported;
+        if ( s.N == 512 )
+            return in_rt == RealT::f32 ? go_r2c_packed_cm<256, float>(s, it, n, max_rows, pre, tw, st)
+                                       : go_r2c_packed_cm<256, __half>(s, it, n, max_rows, pre, tw, st);
+        return in_rt == RealT::f32 ? go_r2c_packed_cm<512, float>(s, it, n, max_rows, pre, tw, st)
+                                   : go_r2c_packed_cm<512, __half>(s, it, n, max_rows, pre, tw, st);
+    }
     {
         hipError_t e = launch_r2c_packed(s, nullptr, nullptr, in_rt, max_rows, pre, tw, it, n, st);
         if ( e != hipErrorNotSupported )

@@ -50,7 +50,16 @@ static hipError_t launch_c2r_packed(const Stage& s, const void* src, void* dst, 
 
 // Template-matching statistics of a chunk (k_c2r_stats_packed) and their reset
 // (a fill kernel, so that a captured graph holds no memset).
-bool c2r_stats_covers(const Stage& s) { return c2r_packed_covers(s) && s.im.n2 == 1 && s.im.es == 1 && s.om.n2 == 1; }
+// Column-major batch input: row r at r, element k at k * es (es even, every column
+// holding whole groups of kCmRows rows).
+static bool c2r_cm_input(const Stage& s) {
+    return s.im.n2 == 1 && s.im.s1 == 1 && s.im.es > 1 && s.im.es % 2 == 0 &&
+           s.im.es >= (s.nlines + kCmRows - 1) / kCmRows * kCmRows;
+}
+
+bool c2r_stats_covers(const Stage& s) {
+    return c2r_packed_covers(s) && s.im.n2 == 1 && (s.im.es == 1 || c2r_cm_input(s)) && s.om.n2 == 1;
+}
 
 // Partial maps of template group g are merged in group order, after the maps
 // of earlier chunks and calls, so the first maximum keeps its template.
@@ -92,9 +101,16 @@ hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, floa
     G             = (n + per - 1) / per; // no empty group
     const dim3  grid((unsigned)((s.nlines + kPackedRows - 1) / kPackedRows), (unsigned)G), block(64 * kPackedRows);
     const auto* it = static_cast<const RowBatchItem*>(items);
+    const bool  cm = c2r_cm_input(s);
     auto        go = [&](auto pc, auto hc) {
-        k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value><<<grid, block, 0, st>>>(
-                to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq, bs.part);
+        if ( cm )
+            k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value, true><<<grid, block, 0, st>>>(
+                    to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq,
+                    bs.part);
+        else
+            k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value><<<grid, block, 0, st>>>(
+                    to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq,
+                    bs.part);
         return hipGetLastError( );
     };
     const bool half = 2 * s.M <= s.N;
@@ -284,8 +300,19 @@ static int c2r_rows_rotation(const Stage& s) {
 }
 
 bool c2r_rows_batch_covers(const Stage& s) {
+    if ( c2r_cm_input(s) )
+        return c2r_packed_covers(s) && s.nlines <= kArgmaxPartials && s.om.n2 == 1;
     return (c2r_rows_rotation(s) >= 0 || c2r_packed_covers(s)) && s.nlines <= kArgmaxPartials && s.im.n2 == 1 &&
            s.om.n2 == 1;
+}
+
+template <int P, typename TOut>
+static hipError_t go_c2r_packed_cm(const Stage& s, const RowBatchItem* items, int n, float post, const float2* tw,
+                                   hipStream_t st) {
+    const dim3 grid((unsigned)((s.nlines + kCmRows - 1) / kCmRows), (unsigned)n);
+    k_c2r_rows_packed_cm<P, TOut><<<grid, dim3(64 * kPackedRows), 0, st>>>(s.im.es, to_device(s.om), s.M, s.o, s.nlines,
+                                                                          post, tw, items);
+    return hipGetLastError( );
 }
 
 hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,
@@ -293,6 +320,13 @@ hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT
     if ( ! c2r_rows_batch_covers(s) )
         return hipErrorNotSupported;
     const auto* it = static_cast<const RowBatchItem*>(items);
+    if ( c2r_cm_input(s) ) {
+        if ( s.N == 512 )
+            return out_rt == RealT::f32 ? go_c2r_packed_cm<256, float>(s, it, n, post, tw, st)
+                                        : go_c2r_packed_cm<256, __half>(s, it, n, post, tw, st);
+        return out_rt == RealT::f32 ? go_c2r_packed_cm<512, float>(s, it, n, post, tw, st)
+                                    : go_c2r_packed_cm<512, __half>(s, it, n, post, tw, st);
+    }
     {
         hipError_t e = launch_c2r_packed(s, nullptr, nullptr, out_rt, post, tw, nullptr, it, n, st);
         if ( e != hipErrorNotSupported )

@@ -100,10 +100,12 @@ bool xcorr_col_covers(const Stage& s, CpxT img_ct) {
 }
 
 bool xcorr_toeplitz_covers(const Stage& s) {
-    return s.kind == StageKind::xcorr && s.N == kTzFy && s.L >= 1 && s.L <= kTzMaxL && s.M >= 1 &&
-           s.M <= kTzMaxM && s.L + s.M - 1 <= s.N && s.o == 0 && s.real_end_lines && s.nlines >= 2 &&
-           single_index_lines(s.im) && s.im.s1 == 1 && single_index_lines(s.om) && s.om.s1 == 1 &&
-           single_index_lines(s.gm) && s.gm.es == 1 && ! env_int("FASTFFT_NO_TOEPLITZ", 0);
+    // column-major template and output slots (rows contiguous, template columns of a
+    // multiple of 16 rows)
+    return s.kind == StageKind::xcorr && s.N == kTzFy && s.L >= 1 && s.L <= kTzMaxL && s.M >= 1 && s.M <= kTzMaxM &&
+           s.L + s.M - 1 <= s.N && s.o == 0 && s.real_end_lines && s.nlines >= 2 && single_index_lines(s.im) &&
+           s.im.es == 1 && s.im.s1 % 16 == 0 && s.im.s1 >= s.L && single_index_lines(s.om) && s.om.es == 1 &&
+           s.om.s1 >= s.M && single_index_lines(s.gm) && s.gm.es == 1 && ! env_int("FASTFFT_NO_TOEPLITZ", 0);
 }
 
 // One workgroup per column (the DC and Nyquist columns share workgroup 0), each running
@@ -111,11 +113,11 @@ bool xcorr_toeplitz_covers(const Stage& s) {
 // products carry N.
 static hipError_t go_xcorr_toeplitz(const Stage& s, const StageIO& io, hipStream_t st) {
     const int n = io.nxitems;
-    if ( ! io.xitems || n < 1 )
+    if ( ! io.xitems || n < 1 || io.xout_step <= 0 )
         return hipErrorNotSupported;
     k_xcorr_toeplitz<<<dim3((unsigned)(s.nlines - 1)), dim3(kTzWaves * 64), 0, st>>>(
-            static_cast<const XcorrItem*>(io.xitems), n, s.im.es, static_cast<const float2*>(io.img), s.gm.s1, s.om.es, s.M,
-            s.L, io.scale * (float)s.N, s.nlines - 1);
+            static_cast<const XcorrItem*>(io.xitems), n, s.im.s1, static_cast<const float2*>(io.img), s.gm.s1, s.om.s1,
+            io.xout_step, s.M, s.L, io.scale * (float)s.N, s.nlines - 1);
     return hipGetLastError( );
 }
 

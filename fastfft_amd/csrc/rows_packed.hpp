@@ -189,6 +189,215 @@ __global__ void __launch_bounds__(64 * kPackedRows)
     }
 }
 
+// Column-major batch slots.  The batched correlation on a reduced frame keeps its
+// intermediates column-major, A[kx][v] and B[kx][j] (column kx of template t is
+// contiguous), so that the correlation passes read and write whole columns.  The
+// row kernels below take kCmRows rows per workgroup (each wave kCmRows / kPackedRows
+// rows in turn) and move them through an LDS tile of kCmRows rows of FFT::SLOTS
+// values (pitch FFT::SLOTS + 2), so that every global access of a column is kCmRows consecutive values
+// (64 bytes) of it: out[k * ldc + row0 + r] / in[k * ldc + row0 + r].  Row r of the
+// tile doubles as the exchange region of the wave that transforms it, so the tile
+// costs no LDS beyond the waves' own regions.
+constexpr int kCmRows = 8;
+
+// k_r2c_rows_packed for batches with column-major output (ldc: elements between
+// columns, even).  Grid: ceil(max rows / kCmRows) x templates.  Rows past the
+// template's row count are stored as zeros up to the end of the workgroup's rows.
+template <int P, typename TIn>
+__global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_per_eu(4)))
+        k_r2c_rows_packed_cm(long ldc, int nout, float pre_scale, const float2* __restrict__ tw,
+                             const RowBatchItem* __restrict__ items) {
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows, RPW = kCmRows / kPackedRows;
+    using FFT       = WaveFFT<P>;
+    // tile row pitch: a row holds X[0 .. P] and the wave's exchange region, and rows start
+    // 16 bytes apart modulo 128 (the column accesses of kCmRows rows hit distinct banks)
+    constexpr int TP = FFT::SLOTS > P + 2 ? FFT::SLOTS : P + 2;
+    static_assert(TP * 8 % 128 == 16, "tile rows 16 bytes apart modulo 128");
+    __shared__ float2 tile[kCmRows * TP];
+    __shared__ float2 s_twp[P];
+    const int          w    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int          t    = threadIdx.x % T;
+    const float2*      twm  = multi_twiddles(tw);
+    const RowBatchItem it   = load_item(items, blockIdx.y);
+    const int          row0 = (int)blockIdx.x * kCmRows;
+    if ( row0 >= it.rows ) // the whole workgroup, before any barrier
+        return;
+    const auto* in = as_global(static_cast<const TIn*>(it.in));
+    // ---- every load of the thread issued before its LDS writes: the pass table and the
+    // wave's rows row0 + w + kPackedRows r
+    float2 ptv[FFT::template pass_table_rows<NT>];
+    FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+    const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
+    float2       v[RPW][E];
+#pragma unroll
+    for ( int r = 0; r < RPW; r++ ) {
+        const int   row = row0 + w + kPackedRows * r;
+        const auto* rp  = in + (long)min(row, it.rows - 1) * it.in_ld;
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int n = 2 * (t + T * i);
+            v[r][i]     = make_float2(n < it.L ? load_real(rp, (long)n) : 0.f, n + 1 < it.L ? load_real(rp, (long)(n + 1)) : 0.f);
+        }
+    }
+    FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+    __syncthreads( );
+    FFT fft;
+    fft.init_pt(t, s_twp);
+#pragma unroll
+    for ( int r = 0; r < RPW; r++ ) {
+        const int  rr   = w + kPackedRows * r; // row within the tile, and the wave's region
+        const bool live = row0 + rr < it.rows;
+        float2*    wl   = tile + rr * TP;
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            v[r][i] = make_float2(v[r][i].x * pre_scale, v[r][i].y * pre_scale);
+        fft.template run<-1>(v[r], wl, t); // v[r][i] = Z[t + 64 i]
+        // Z[P - k] through the region (its last exchange was read before these writes)
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            wl[FFT::Layout::slot(t + T * i)] = v[r][i];
+        wave_lds_fence( );
+        float2 zp[E];
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            zp[i] = wl[FFT::Layout::slot((P - t - T * i) & (P - 1))];
+        wave_lds_fence( ); // every lane's Z reads are done before X overwrites the region
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int    k = t + T * i;
+            const float2 a = make_float2(0.5f * (v[r][i].x + zp[i].x), 0.5f * (v[r][i].y - zp[i].y));
+            const float2 b = make_float2(0.5f * (v[r][i].x - zp[i].x), 0.5f * (v[r][i].y + zp[i].y));
+            const float2 c = cmul(cmul(twm[N + T * i], wt), b);
+            wl[k]          = live ? make_float2(a.x + c.y, a.y - c.x) : make_float2(0.f, 0.f);
+        }
+        if ( t == 0 )
+            wl[P] = live ? make_float2(v[r][0].x - v[r][0].y, 0.f) : make_float2(0.f, 0.f);
+    }
+    __syncthreads( );
+    // ---- columns k < nout: thread pairs store the tile's kCmRows values of a column
+    auto* op = as_global(static_cast<float2*>(it.out)) + row0;
+    for ( int idx = threadIdx.x; idx < 2 * nout; idx += NT ) {
+        const int     k  = idx >> 1, h = idx & 1;
+        const float2* tp = tile + 4 * h * TP + k;
+        store_cpx_pair(op + (long)k * ldc + 4 * h, 0, tp[0], tp[TP]);
+        store_cpx_pair(op + (long)k * ldc + 4 * h, 1, tp[2 * TP], tp[3 * TP]);
+    }
+}
+
+// k_c2r_rows_packed for batches with column-major input (ldc: elements between
+// columns, even; the input column holds at least ceil(nrows / kCmRows) kCmRows
+// values).  Grid: ceil(nrows / kCmRows) x templates.
+template <int P, typename TOut>
+__global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_per_eu(4)))
+        k_c2r_rows_packed_cm(long ldc, LineMap om, int M, int o, int nrows, float post_scale, const float2* __restrict__ tw,
+                             const RowBatchItem* __restrict__ items) {
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows, RPW = kCmRows / kPackedRows;
+    using FFT       = WaveFFT<P>;
+    // tile row pitch: a row holds X[0 .. P] and the wave's exchange region, and rows start
+    // 16 bytes apart modulo 128 (the column accesses of kCmRows rows hit distinct banks)
+    constexpr int TP = FFT::SLOTS > P + 2 ? FFT::SLOTS : P + 2;
+    static_assert(TP * 8 % 128 == 16, "tile rows 16 bytes apart modulo 128");
+    __shared__ float2 tile[kCmRows * TP];
+    __shared__ float2 s_twp[P];
+    const int          w    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int          t    = threadIdx.x % T;
+    const float2*      twm  = multi_twiddles(tw);
+    const RowBatchItem it   = load_item(items, blockIdx.y);
+    const int          row0 = (int)blockIdx.x * kCmRows;
+    if ( row0 >= nrows )
+        return;
+    // ---- the tile: columns k <= P, kCmRows values each, then the pass table
+    {
+        const auto* ip = as_global(static_cast<const float2*>(it.in)) + row0;
+        float2      ptv[FFT::template pass_table_rows<NT>];
+        FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
+        // column halves per thread, in two rounds of loads (fewer registers live)
+        constexpr int NL = (2 * (P + 1) + NT - 1) / NT, NL1 = (NL + 1) / 2;
+#pragma unroll
+        for ( int l0 = 0; l0 < NL; l0 += NL1 ) {
+            float4 a[NL1], b[NL1];
+#pragma unroll
+            for ( int l = 0; l < NL1; l++ ) {
+                const int idx = threadIdx.x + NT * (l0 + l);
+                const int k   = min(idx >> 1, P), h = idx & 1;
+                a[l]          = load_cpx_pair(ip + (long)k * ldc + 4 * h, 0);
+                b[l]          = load_cpx_pair(ip + (long)k * ldc + 4 * h, 1);
+            }
+#pragma unroll
+            for ( int l = 0; l < NL1; l++ ) {
+                const int idx = threadIdx.x + NT * (l0 + l);
+                if ( l0 + l < NL && idx < 2 * (P + 1) ) {
+                    float2* tp = tile + 4 * (idx & 1) * TP + (idx >> 1);
+                    tp[0]      = make_float2(a[l].x, a[l].y);
+                    tp[TP]     = make_float2(a[l].z, a[l].w);
+                    tp[2 * TP] = make_float2(b[l].x, b[l].y);
+                    tp[3 * TP] = make_float2(b[l].z, b[l].w);
+                }
+            }
+        }
+        FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
+    }
+    __syncthreads( );
+    const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
+    FFT fft;
+    fft.init_pt(t, s_twp);
+    const int oN = ((o % N) + N) % N;
+#pragma unroll
+    for ( int r = 0; r < RPW; r++ ) {
+        const int rr  = w + kPackedRows * r;
+        const int row = row0 + rr;
+        if ( row >= nrows ) // wave-uniform; no barrier follows
+            break;
+        // the row into registers, then its tile row serves as the transform's exchange region
+        // (only this wave reads or writes it)
+        float2        vr[E];
+        float2*       tr = tile + rr * TP;
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int k  = t + T * i;
+            float2    x  = tr[k];
+            float2    xp = tr[P - k];
+            if ( k == 0 ) { // the real part of the Hermitian extension's inverse ignores Im X[0], Im X[P]
+                x.y  = 0.f;
+                xp.y = 0.f;
+            }
+            const float2 e = make_float2(x.x + xp.x, x.y - xp.y);
+            const float2 d = make_float2(x.x - xp.x, x.y + xp.y);
+            const float2 c = cmulc(d, cmul(twm[N + T * i], wt));
+            vr[i]          = make_float2(e.x - c.y, e.y + c.x);
+        }
+        wave_lds_fence( ); // every lane's reads of the row are done before the transform writes it
+        fft.template run<+1>(vr, tr, t); // vr[i] = z[t + 64 i]
+        auto*              op   = as_global(static_cast<TOut*>(it.out)) + om.base(row);
+        unsigned long long best = 0;
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int   n    = 2 * (t + T * i);
+            const float y[2] = {vr[i].x * post_scale, vr[i].y * post_scale};
+#pragma unroll
+            for ( int h = 0; h < 2; h++ ) {
+                const int j = (n + h - oN + N) & (N - 1);
+                if ( j < M ) {
+                    store_real(op, (long)j * om.es, y[h]);
+                    if ( it.keys ) {
+                        const auto k = argmax_key(convert_real<TOut>(y[h]), (unsigned)(row * M + j));
+                        best         = k > best ? k : best;
+                    }
+                }
+            }
+        }
+        if ( it.keys ) {
+#pragma unroll
+            for ( int off = 32; off > 0; off >>= 1 ) {
+                const unsigned long long other = __shfl_xor(best, off);
+                best                           = other > best ? other : best;
+            }
+            if ( t == 0 )
+                as_global(it.keys)[row] = best;
+        }
+    }
+}
+
 // Template-matching statistics over a chunk of templates (no counterpart in the
 // reference; the per-pixel consumer of a 2DTM search): the wave of row r runs
 // the C2R of that row for items[0 .. nitems) in order and keeps, for each
@@ -204,7 +413,9 @@ __global__ void __launch_bounds__(64 * kPackedRows)
 // takes items [g per, (g + 1) per) and writes its statistics to its own
 // partial maps part + 4 g npix (mip, best, sum, sum_sq as 32-bit words) for
 // k_stats_merge.
-template <int P, bool HALF>
+// CM: column-major input (element k of row r at k * im.es + r, im.es even): per item the
+// workgroup first copies its kPackedRows rows' columns into an LDS tile.
+template <int P, bool HALF, bool CM = false>
 __global__ void __launch_bounds__(64 * kPackedRows)
         k_c2r_stats_packed(LineMap im, int M, int o, int nrows, float post_scale, const float2* __restrict__ tw,
                            const RowBatchItem* __restrict__ items, int nitems, int first_id, float* __restrict__ mip,
@@ -224,8 +435,10 @@ __global__ void __launch_bounds__(64 * kPackedRows)
         sum_sq = mip + 3 * npix;
     }
     using FFT       = WaveFFT<P>;
+    constexpr int CP = kPackedRows + 1; // tile pitch (CM)
     __shared__ float2 lds[kPackedRows * FFT::SLOTS];
     __shared__ float2 s_twp[P];
+    __shared__ float2 tile[CM ? (P + 1) * CP : 1];
     const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int     t   = threadIdx.x % T;
     const float2* twm = multi_twiddles(tw);
@@ -235,7 +448,7 @@ __global__ void __launch_bounds__(64 * kPackedRows)
     const bool live = row < nrows;
     FFT::template fill_pass_table<NT>(s_twp, twm, threadIdx.x);
     __syncthreads( );
-    if ( ! live )
+    if ( ! CM && ! live ) // (CM: every wave takes part in the tile barriers)
         return;
     const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
     FFT          fft;
@@ -254,17 +467,43 @@ __global__ void __launch_bounds__(64 * kPackedRows)
             lsum[r][h] = lsq[r][h] = 0.f;
         }
     for ( int it = 0; it < nitems; it++ ) {
-        // unit-stride rows (im.es == 1): X[t + 64 i] and X[P - t - 64 i] at constant
-        // offsets from two lane pointers
         const RowBatchItem item = load_item(items, it);
-        const auto*        ip   = as_global(static_cast<const float2*>(item.in)) + im.base(row);
-        const auto*        lo   = ip + t;
-        const auto*        hi   = ip + (P - t);
         float2             x[E], xp[E];
+        if constexpr ( CM ) {
+            // the workgroup's rows: kPackedRows consecutive values of every column k <= P
+            const int   r0 = (int)blockIdx.x * kPackedRows;
+            const auto* ip = as_global(static_cast<const float2*>(item.in)) + r0;
+            __syncthreads( ); // the previous item's tile reads are done
+            for ( int k = threadIdx.x; k < P + 1; k += NT ) {
+                const float4 a  = load_cpx_pair(ip + (long)k * im.es, 0);
+                const float4 b  = load_cpx_pair(ip + (long)k * im.es, 1);
+                float2*      tp = tile + k * CP;
+                tp[0]           = make_float2(a.x, a.y);
+                tp[1]           = make_float2(a.z, a.w);
+                tp[2]           = make_float2(b.x, b.y);
+                tp[3]           = make_float2(b.z, b.w);
+            }
+            __syncthreads( );
+            if ( ! live )
+                continue;
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
-            x[i]  = load_cpx(lo, T * i);
-            xp[i] = load_cpx(hi, -T * i);
+            for ( int i = 0; i < E; i++ ) {
+                const int k = t + T * i;
+                x[i]        = tile[k * CP + w];
+                xp[i]       = tile[(P - k) * CP + w];
+            }
+        }
+        else {
+            // unit-stride rows (im.es == 1): X[t + 64 i] and X[P - t - 64 i] at constant
+            // offsets from two lane pointers
+            const auto* ip = as_global(static_cast<const float2*>(item.in)) + im.base(row);
+            const auto* lo = ip + t;
+            const auto* hi = ip + (P - t);
+#pragma unroll
+            for ( int i = 0; i < E; i++ ) {
+                x[i]  = load_cpx(lo, T * i);
+                xp[i] = load_cpx(hi, -T * i);
+            }
         }
         if ( t == 0 ) {
             x[0].y  = 0.f;
@@ -303,6 +542,8 @@ __global__ void __launch_bounds__(64 * kPackedRows)
         }
         wave_lds_fence( ); // the window reads are done before the next transform writes the region
     }
+    if ( ! live ) // (CM: the waves past the last row stayed for the tile barriers)
+        return;
     auto* gm = as_global(mip);
     auto* gb = as_global(best);
     auto* gs = as_global(sum);

@@ -92,9 +92,9 @@ struct TzShared {
 };
 
 template <bool ENDS>
-__device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restrict__ items, int nitems, long a_es,
-                                        const float2* __restrict__ S, long s_stride, long o_es, int M, int Lmax,
-                                        float scale, int last, int col) {
+__device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restrict__ items, int nitems, long lda,
+                                        const float2* __restrict__ S, long s_stride, long ldo, long out_step, int M,
+                                        int Lmax, float scale, int last, int col) {
     const int tid     = threadIdx.x;
     const int lane    = tid & 63;
     const int w       = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -116,20 +116,26 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
         const int       grp = q / nchk, ck = q % nchk;
         const int       tt  = grp * kTzTemplates + tn;
         const XcorrItem it  = items[tt < nitems ? tt : 0];
-        const int       Lc  = it.L > 0 ? it.L : 1;
         const auto*     ga  = as_global(static_cast<const float2*>(it.a));
-        const int       v0  = 16 * (ck * kTzChunk + tk);
-        // element offsets in 32 bits (a template column spans at most 512 rows of a_es
-        // elements), computed per load so that none is kept across the chunk
-        int es = (int)a_es;
-        asm volatile("" : "+s"(es));
+        // 16 contiguous rows of the template's column (the column holds lda rows, a
+        // multiple of 16; K-steps past it read its last 16 rows, which the staging zeroes);
+        // 32-bit offsets, formed here so that none is kept across the chunk
+        int ld = (int)lda;
+        asm volatile("" : "+s"(ld));
+        const int v0 = min(16 * (ck * kTzChunk + tk), ld - 16);
 #pragma unroll
-        for ( int i = 0; i < 16; i++ ) {
-            const int r = min(v0 + i, Lc - 1) * es;
-            if constexpr ( ENDS )
-                z[i] = make_float2(load_cpx(ga, r).x, load_cpx(ga, r + last).x);
-            else
-                z[i] = load_cpx(ga, r + col);
+        for ( int i = 0; i < 8; i++ ) {
+            if constexpr ( ENDS ) {
+                const float4 a = load_cpx_pair(ga + v0, i);
+                const float4 b = load_cpx_pair(ga + last * ld + v0, i);
+                z[2 * i]       = make_float2(a.x, b.x);
+                z[2 * i + 1]   = make_float2(a.z, b.z);
+            }
+            else {
+                const float4 a = load_cpx_pair(ga + col * ld + v0, i);
+                z[2 * i]       = make_float2(a.x, a.y);
+                z[2 * i + 1]   = make_float2(a.z, a.w);
+            }
         }
     };
     auto store_chunk = [&](int q, int into) {
@@ -285,7 +291,7 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
                     for ( int p = 0; p < 3; p++ )
 #pragma unroll
                         for ( int a = 0; a < kTzTiles; a++ )
-                            acc[a][p] = tz_mfma(aslot[(i + a) % 5][p][ha], bop[i & 1][p][hb], acc[a][p]);
+                            acc[a][p] = tz_mfma(bop[i & 1][p][hb], aslot[(i + a) % 5][p][ha], acc[a][p]);
                 }
                 // issue order of the K-step: the next K-step's twelve LDS reads two at a time
                 // between the first six matrix instructions (read a full K-step ahead, not
@@ -309,39 +315,34 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
             __syncthreads( );
       }
       {
-            // ---- the group's outputs: register r of tile a holds row
-            // 128 w + 32 a + 8 (r / 4) + 4 g + r % 4 of template n of the group
-            const int tout = grp * kTzTemplates + n;
-            if ( tout < nitems ) {
-                // 32-bit element offsets: the row steps are wave-uniform multiples of o_es,
-                // formed here (the empty asm keeps the compiler from hoisting 64 of them out
-                // of the chunk loop, where they would not fit in registers)
-                int oes = (int)o_es, mm = M;
-                asm volatile("" : "+s"(oes), "+s"(mm));
-                auto*     go  = as_global(static_cast<float2*>(items[tout].out));
+            // ---- the group's outputs: register r of tile a holds template 8 (r / 4) + 4 g + r % 4
+            // of the group at output row 128 w + 32 a + n, so each store covers 32 consecutive rows
+            // of a template's column (the templates' outputs are out_step elements apart)
+            int oldo = (int)ldo, mm = M, ostep = (int)out_step;
+            asm volatile("" : "+s"(oldo), "+s"(mm), "+s"(ostep));
+            auto* go = as_global(static_cast<float2*>(items[0].out)) + (grp * kTzTemplates + 4 * g) * (long)ostep;
 #pragma unroll
-                for ( int a = 0; a < kTzTiles; a++ ) {
-                    const int rowa = kTzRowsWave * w + 32 * a;
-                    if ( rowa < mm ) {
-                        auto* rp = go + (rowa + 4 * g) * oes;
+            for ( int a = 0; a < kTzTiles; a++ ) {
+                const int j = kTzRowsWave * w + 32 * a + n;
+                if ( j < mm ) {
 #pragma unroll
-                        for ( int r = 0; r < 16; r++ ) {
-                            const int dj = 8 * (r >> 2) + (r & 3);
-                            if ( rowa + 4 * g + dj < mm ) {
-                                const float k0 = acc[a][0][r], k1 = acc[a][1][r], k2 = acc[a][2][r];
-                                if constexpr ( ! ENDS ) {
-                                    store_cpx(rp, dj * oes + col, make_float2((k0 - k2) * scale, (k0 + k1) * scale));
-                                }
-                                else {
-                                    store_cpx(rp, dj * oes, make_float2(k0 * scale, 0.f));
-                                    store_cpx(rp, dj * oes + last, make_float2(k1 * scale, 0.f));
-                                }
+                    for ( int r = 0; r < 16; r++ ) {
+                        const int tl = 8 * (r >> 2) + (r & 3); // template of the group, less 4 g
+                        if ( grp * kTzTemplates + 4 * g + tl < nitems ) {
+                            const float k0 = acc[a][0][r], k1 = acc[a][1][r], k2 = acc[a][2][r];
+                            auto*       to = go + tl * ostep;
+                            if constexpr ( ! ENDS ) {
+                                store_cpx(to, col * oldo + j, make_float2((k0 - k2) * scale, (k0 + k1) * scale));
+                            }
+                            else {
+                                store_cpx(to, j, make_float2(k0 * scale, 0.f));
+                                store_cpx(to, last * oldo + j, make_float2(k1 * scale, 0.f));
                             }
                         }
                     }
-                    // one tile's accumulators at a time out of the matrix registers
-                    __builtin_amdgcn_sched_barrier(0);
                 }
+                // one tile's accumulators at a time out of the matrix registers
+                __builtin_amdgcn_sched_barrier(0);
             }
       }
     }
@@ -349,21 +350,23 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
 
 // Grid: x = last workgroups (columns 1 .. last - 1, and workgroup 0 for the pair
 // 0 / last); 256 threads; every workgroup runs all ceil(nitems / 32) template groups.
-//   S       frame image in mixed form, column c at S + c * s_stride (Fy = 1024 values)
-//   items   the templates: a[v * a_es + c] element v of column c of template t
-//           (1 <= L_t <= Lmax rows), out[j * o_es + c] element j of its output column
-//   M       output rows (<= 512), Lmax the largest template's rows (<= 512, with
-//           Lmax + M - 1 <= 1024), last the Nyquist column index (ncols - 1)
+//   S        frame image in mixed form, column c at S + c * s_stride (Fy = 1024 values)
+//   items    the templates, column-major: a[c * lda + v] element v of column c of template
+//            t (1 <= L_t <= Lmax rows; lda a multiple of 16, at least Lmax), and
+//            out[c * ldo + j] element j of its output column; the outputs of the launch's
+//            templates are out_step elements apart (items[t].out = items[0].out + t out_step)
+//   M        output rows (<= 512), Lmax the largest template's rows (<= 512, with
+//            Lmax + M - 1 <= 1024), last the Nyquist column index (ncols - 1)
 __global__ void __launch_bounds__(kTzWaves * 64, 1)
-        k_xcorr_toeplitz(const XcorrItem* __restrict__ items, int nitems, long a_es, const float2* __restrict__ S,
-                         long s_stride, long o_es, int M, int Lmax, float scale, int last) {
+        k_xcorr_toeplitz(const XcorrItem* __restrict__ items, int nitems, long lda, const float2* __restrict__ S,
+                         long s_stride, long ldo, long out_step, int M, int Lmax, float scale, int last) {
     __shared__ TzShared sh;
     // each XCD takes a contiguous range of columns (template and output lines shared in one L2)
     const int col = __builtin_amdgcn_readfirstlane(xcd_row(blockIdx.x, gridDim.x));
     if ( col == 0 ) // workgroup 0 takes columns 0 and last
-        tz_body<true>(sh, items, nitems, a_es, S, s_stride, o_es, M, Lmax, scale, last, col);
+        tz_body<true>(sh, items, nitems, lda, S, s_stride, ldo, out_step, M, Lmax, scale, last, col);
     else
-        tz_body<false>(sh, items, nitems, a_es, S, s_stride, o_es, M, Lmax, scale, last, col);
+        tz_body<false>(sh, items, nitems, lda, S, s_stride, ldo, out_step, M, Lmax, scale, last, col);
 }
 
 } // namespace fastfft_amd
