@@ -1631,7 +1631,9 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
                              : plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? std::string("fp32 (split_bf16 does not apply)")
                                                                                : std::string("fp32"));
         if ( plan->sub_active ) {
-            s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) + "\n";
+            const bool cm = plan->sub_xc.size( ) == 3 && plan->sub_xc[1].im.es == 1 && plan->sub_xc[1].im.s1 > 1;
+            s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) +
+                 (cm ? " slots=column-major" : " slots=row-major") + "\n";
             s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +
                  "templates:\n" + describe(plan->st, plan->sub_xc);
         }

@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(64 * kPackedRows)
 // row kernels below take kCmRows rows per workgroup (each wave kCmRows / kPackedRows
 // rows in turn) and move them through an LDS tile of kCmRows rows of FFT::SLOTS
 // values (pitch FFT::SLOTS + 2), so that every global access of a column is kCmRows consecutive values
-// (64 bytes) of it: out[k * ldc + row0 + r] / in[k * ldc + row0 + r].  Row r of the
+// (64 bytes) of it, taken by four lanes of 16 bytes: out[k * ldc + row0 + r] / in[k * ldc + row0 + r].  Row r of the
 // tile doubles as the exchange region of the wave that transforms it, so the tile
 // costs no LDS beyond the waves' own regions.
 constexpr int kCmRows = 8;
@@ -276,11 +276,11 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
     __syncthreads( );
     // ---- columns k < nout: thread pairs store the tile's kCmRows values of a column
     auto* op = as_global(static_cast<float2*>(it.out)) + row0;
-    for ( int idx = threadIdx.x; idx < 2 * nout; idx += NT ) {
-        const int     k  = idx >> 1, h = idx & 1;
-        const float2* tp = tile + 4 * h * TP + k;
-        store_cpx_pair(op + (long)k * ldc + 4 * h, 0, tp[0], tp[TP]);
-        store_cpx_pair(op + (long)k * ldc + 4 * h, 1, tp[2 * TP], tp[3 * TP]);
+    constexpr int HQ = kCmRows / 2; // lanes per column, 2 rows (16 bytes) each
+    for ( int idx = threadIdx.x; idx < HQ * nout; idx += NT ) {
+        const int     k  = idx / HQ, h = idx % HQ;
+        const float2* tp = tile + 2 * h * TP + k;
+        store_cpx_pair(op + (long)k * ldc + 2 * h, 0, tp[0], tp[TP]);
     }
 }
 
@@ -311,27 +311,26 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
         const auto* ip = as_global(static_cast<const float2*>(it.in)) + row0;
         float2      ptv[FFT::template pass_table_rows<NT>];
         FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
-        // column halves per thread, in two rounds of loads (fewer registers live)
-        constexpr int NL = (2 * (P + 1) + NT - 1) / NT, NL1 = (NL + 1) / 2;
+        // lanes per column HQ, 2 rows (16 bytes) each, so that four lanes take a column's 64
+        // bytes; in two rounds of loads (fewer registers live)
+        constexpr int HQ = kCmRows / 2;
+        constexpr int NL = (HQ * (P + 1) + NT - 1) / NT, NL1 = (NL + 1) / 2;
 #pragma unroll
         for ( int l0 = 0; l0 < NL; l0 += NL1 ) {
-            float4 a[NL1], b[NL1];
+            float4 a[NL1];
 #pragma unroll
             for ( int l = 0; l < NL1; l++ ) {
                 const int idx = threadIdx.x + NT * (l0 + l);
-                const int k   = min(idx >> 1, P), h = idx & 1;
-                a[l]          = load_cpx_pair(ip + (long)k * ldc + 4 * h, 0);
-                b[l]          = load_cpx_pair(ip + (long)k * ldc + 4 * h, 1);
+                const int k   = min(idx / HQ, P), h = idx % HQ;
+                a[l]          = load_cpx_pair(ip + (long)k * ldc + 2 * h, 0);
             }
 #pragma unroll
             for ( int l = 0; l < NL1; l++ ) {
                 const int idx = threadIdx.x + NT * (l0 + l);
-                if ( l0 + l < NL && idx < 2 * (P + 1) ) {
-                    float2* tp = tile + 4 * (idx & 1) * TP + (idx >> 1);
+                if ( l0 + l < NL && idx < HQ * (P + 1) ) {
+                    float2* tp = tile + 2 * (idx % HQ) * TP + idx / HQ;
                     tp[0]      = make_float2(a[l].x, a[l].y);
                     tp[TP]     = make_float2(a[l].z, a[l].w);
-                    tp[2 * TP] = make_float2(b[l].x, b[l].y);
-                    tp[3 * TP] = make_float2(b[l].z, b[l].w);
                 }
             }
         }

@@ -60,7 +60,8 @@ FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "
           ((2048, 2048), (512, 400)): "1024x1024", ((8192, 1024), (512, 64)): "1024x512"}
 
 
-@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "no_frame_fusion", "full_frame", "per_template"])
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major", "full_frame",
+                                  "per_template"])
 @pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
 def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
     """Batched kernels (chunk size from the tuner, or 2 to force several chunks, or the two-wave row
@@ -77,11 +78,18 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "full_frame":
         monkeypatch.setenv("FASTFFT_NO_SUBFRAME", "1")
+    if mode == "row_major":
+        monkeypatch.setenv("FASTFFT_BATCH_ROW_MAJOR", "1")
     H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
     form = ft.Describe("batch").splitlines()[0]
-    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed", "no_frame_fusion") else None
+    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major") else None
     assert f"frame={frame or 'full'}" in form, form
+    if frame:
+        # the frame's template and output slots: column-major where the tiled row kernels cover the
+        # frame's rows (512- and 1024-point rows), row-major when asked for
+        cm = mode != "row_major" and frame.split("x")[0] in ("512", "1024") and mode != "no_packed"
+        assert ("slots=column-major" if cm else "slots=row-major") in form, form
     ft.FwdImageInvFFTBatch(img)
     pk, vv = H.to_host(peaks).reshape(-1, 2), H.to_host(vals)
     pitch = H.F.padded_pitch(nout[0])
@@ -97,7 +105,7 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         assert tuple(pk[i]) == (rx, ry) and vv[i] == np.float32(rv), i
         # the FP64 oracle of the full N-frame pipeline: every small batch, and the batches on a reduced
         # frame in the modes that run the frame (the fused frame rows at Q = 2, 4 and 8 included)
-        if n[0] * n[1] <= 256 * 128 or (frame and mode in ("tuned", "no_packed", "no_frame_fusion")):
+        if n[0] * n[1] <= 256 * 128 or (frame and mode in ("tuned", "no_packed", "no_frame_fusion", "row_major")):
             want = O.xcorr_2d(H.padded_real(hosts[i], x), x, spec, n[0], n[1], nout[0], nout[1], *origin)
             assert O.rel_l2(got, want) < TOL, i
             wx, wy, _ = O.argmax_2d(want)
@@ -256,7 +264,7 @@ def _planted_mixed(n, m, sizes, seed):
     return mic, tmpls, lags
 
 
-@pytest.mark.parametrize("chunk", [0, 4])
+@pytest.mark.parametrize("chunk", [0, 4, "row_major"])
 def test_headline_batch_4096_mixed_against_oracle(chunk, monkeypatch):
     """The headline form: FP32 templates of 128^2, 256^2 and 384^2 (two of each, circular masks, each planted
     once in a 4096^2 micrograph at a lag inside the window) correlated by one batched call on the reduced
@@ -265,7 +273,10 @@ def test_headline_batch_4096_mixed_against_oracle(chunk, monkeypatch):
     runs the batch in two chunks of launches."""
     from fastfft_amd import workloads as W
 
-    if chunk:
+    if chunk == "row_major":
+        monkeypatch.setenv("FASTFFT_BATCH_ROW_MAJOR", "1")
+        chunk = 0
+    elif chunk:
         monkeypatch.setenv("FASTFFT_BATCH_CHUNK", str(chunk))
     n, m = 4096, 512
     H.log(f"headline batch 4096^2 x (128, 256, 384)^2 -> 512^2 against the FP64 oracle, chunk={chunk or 'tuned'}")
