@@ -24,24 +24,27 @@
 // U1 = Re S_last, X0 = a_0, X1 = a_last (U2 = X2 = 0) give them as k0 and k1.
 //
 // Workgroup: one column (or the DC / Nyquist pair) and every template group of
-// the launch in turn; four waves, one per SIMD, wave w owns output rows
-// [128 w, 128 w + 128) as four 32-row tiles, for all 32 templates of a group.
-//   * Left operands: U_p split into hi / lo bf16, staged once per workgroup in
-//     LDS in four copies shifted by 0..3 elements, so that every lane's
-//     operand (8 consecutive elements from row j + 16 k + 8 g) is one 16-byte
-//     read at an 8-byte aligned index of copy j % 4; the copies start 32 bytes
-//     apart modulo 128, so the reads of sixteen lanes cover 128 distinct bytes.
+// the launch in turn; eight waves, two per SIMD, wave w owns output rows
+// [64 w, 64 w + 64) as two 32-row tiles, for all 32 templates of a group.  The
+// templates are the matrix instruction's A operand and the Toeplitz rows its B
+// operand, so that a result register holds 32 consecutive output rows of one
+// template: every store covers whole column segments of the column-major outputs.
+//   * Left operands (the Toeplitz B operand): U_p split into hi / lo bf16, staged
+//     once per workgroup in LDS in four copies shifted by 0..3 elements, so that
+//     every lane's operand (8 consecutive elements from row j + 16 k + 8 g) is one
+//     16-byte read at an 8-byte aligned index of copy j % 4; the copies start 32
+//     bytes apart modulo 128, so the reads of sixteen lanes cover 128 distinct bytes.
 //   * Toeplitz reuse: tile a's operand at K-step k is tile 0's at K-step
 //     k + 2 a.  A chunk's even K-steps run first, then its odd ones; in each
-//     pass a wave keeps tile 0's operands of five K-steps of one parity in
+//     pass a wave keeps tile 0's operands of three K-steps of one parity in
 //     registers and reads one new operand per K-step and (p, hi / lo), one
-//     K-step ahead, after four at the start of the pass (7 reads per 4 K-steps
-//     where four tiles would take 16).
-//   * Right operands: the templates' X_p, split on the way into LDS, in chunks
-//     of eight K-steps, double buffered: the global loads of the next chunk are
-//     in flight while the waves run the current one.  The operands of the next
-//     K-step are read from LDS while the current K-step's 36 matrix
-//     instructions run.
+//     K-step ahead, after two at the start of the pass.
+//   * Right operands (the templates, A operand): the templates' X_p, read as
+//     contiguous column segments and split on the way into LDS, in chunks of
+//     eight K-steps, double buffered: the global loads of the next chunk are in
+//     flight while the waves run the current one.  The operands of the next
+//     K-step are read from LDS while the current K-step's 18 matrix instructions
+//     run.
 //
 // MFMA lane layouts (v_mfma_f32_32x32x16_bf16, checked on the device):
 //   A: lane l holds A[l % 32][8 (l / 32) + e], e < 8
@@ -59,8 +62,8 @@ typedef __attribute__((ext_vector_type(16))) float tz_f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned tz_u32x4;
 typedef __attribute__((ext_vector_type(2))) unsigned tz_u32x2;
 
-constexpr int kTzWaves     = 4;                    // waves per workgroup (one per SIMD)
-constexpr int kTzTiles     = 4;                    // 32-row tiles per wave
+constexpr int kTzWaves     = 8;                    // waves per workgroup (two per SIMD)
+constexpr int kTzTiles     = 2;                    // 32-row tiles per wave
 constexpr int kTzRowsWave  = 32 * kTzTiles;        // output rows per wave
 constexpr int kTzMaxM      = kTzWaves * kTzRowsWave;
 constexpr int kTzTemplates = 32;                   // templates per group (the MFMA's N)
@@ -106,25 +109,30 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
     const int nq      = ngroups * nchk; // staged chunks over all groups
 
     // ---- staging of the right operands: thread tid takes template tn = tid % 32 of the
-    // group and K-step tk = tid / 32 of the chunk (rows 16 tk .. 16 tk + 15).  Every load is
-    // issued (rows clamped to the template, threads past the batch on template 0) and the
-    // values outside the template replaced by zeros, so no load waits for another.
+    // group and ZR rows of the chunk (ZR = 16: K-step tk = tid / 32, both halves; ZR = 8:
+    // K-step tk / 2, half tk % 2).  Every load is issued (threads past the batch on template
+    // 0) and the values outside the template replaced by zeros, so no load waits for another.
+    constexpr int NT = kTzWaves * 64;
+    constexpr int ZR = kTzChunk * 16 * kTzTemplates / NT; // template rows per thread and chunk
+    static_assert(ZR == 16 || ZR == 8, "a thread stages one K-step or one half of it");
     const int tn = tid & 31;
     const int tk = tid >> 5;
-    float2    z[16];
+    const int zk = ZR == 16 ? tk : tk >> 1;           // K-step of the chunk
+    const int zh = ZR == 16 ? 0 : tk & 1;             // first half
+    float2    z[ZR];
     auto      load_chunk = [&](int q) {
         const int       grp = q / nchk, ck = q % nchk;
         const int       tt  = grp * kTzTemplates + tn;
         const XcorrItem it  = items[tt < nitems ? tt : 0];
         const auto*     ga  = as_global(static_cast<const float2*>(it.a));
-        // 16 contiguous rows of the template's column (the column holds lda rows, a
-        // multiple of 16; K-steps past it read its last 16 rows, which the staging zeroes);
-        // 32-bit offsets, formed here so that none is kept across the chunk
+        // ZR contiguous rows of the template's column (the column holds lda rows, a multiple
+        // of 16; K-steps past it read its last 16 rows, which the staging zeroes); 32-bit
+        // offsets, formed here so that none is kept across the chunk
         int ld = (int)lda;
         asm volatile("" : "+s"(ld));
-        const int v0 = min(16 * (ck * kTzChunk + tk), ld - 16);
+        const int v0 = min(16 * (ck * kTzChunk + zk), ld - 16) + 8 * zh;
 #pragma unroll
-        for ( int i = 0; i < 8; i++ ) {
+        for ( int i = 0; i < ZR / 2; i++ ) {
             if constexpr ( ENDS ) {
                 const float4 a = load_cpx_pair(ga + v0, i);
                 const float4 b = load_cpx_pair(ga + last * ld + v0, i);
@@ -142,15 +150,16 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
         const int grp = q / nchk, ck = q % nchk;
         const int tt  = grp * kTzTemplates + tn;
         const int Lt  = tt < nitems ? items[tt].L : 0; // rows taken
-        const int v0  = 16 * (ck * kTzChunk + tk);
-        auto&     dst = sh.x[into][tk];
+        const int v0  = 16 * (ck * kTzChunk + zk);
+        auto&     dst = sh.x[into][zk];
 #pragma unroll
-        for ( int gg = 0; gg < 2; gg++ ) {
-            unsigned hw[3][8], lw[3][8];
+        for ( int hh = 0; hh < ZR / 8; hh++ ) {
+            const int gg = zh + hh;
+            unsigned  hw[3][8], lw[3][8];
 #pragma unroll
             for ( int i = 0; i < 8; i++ ) {
                 const bool   in = v0 + 8 * gg + i < Lt;
-                const float2 a  = in ? z[8 * gg + i] : make_float2(0.f, 0.f);
+                const float2 a  = in ? z[8 * hh + i] : make_float2(0.f, 0.f);
                 float        x0, x1, x2;
                 if constexpr ( ENDS ) {
                     x0 = a.x, x1 = a.y, x2 = 0.f;
@@ -177,15 +186,17 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
     };
     load_chunk(0);
 
-    // ---- left operands: thread tid writes indices 4 tid .. 4 tid + 3 of every copy, from
-    // U_p[4 tid .. 4 tid + 6] (zero past the frame)
+    // ---- left operands: thread tid writes indices YT tid .. YT tid + YT - 1 of every copy,
+    // from U_p[YT tid .. YT tid + YT + 2] (zero past the frame)
     {
+        constexpr int YT = kTzFy / NT; // y values per thread
+        static_assert(YT == 4 || YT == 2, "8- or 4-byte copy writes");
         const auto* s0 = as_global(S) + (long)col * s_stride;
         const auto* s1 = as_global(S) + (long)last * s_stride;
-        const int   y0 = 4 * tid;
-        unsigned    uh[3][7], ul[3][7];
+        const int   y0 = YT * tid;
+        unsigned    uh[3][YT + 3], ul[3][YT + 3];
 #pragma unroll
-        for ( int i = 0; i < 7; i++ ) {
+        for ( int i = 0; i < YT + 3; i++ ) {
             const int y = y0 + i;
             float     u0 = 0.f, u1 = 0.f, u2 = 0.f;
             if ( y < kTzFy ) {
@@ -206,10 +217,16 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
         for ( int p = 0; p < 3; p++ )
 #pragma unroll
             for ( int c = 0; c < 4; c++ ) {
-                const tz_u32x2 h = {uh[p][c] | (uh[p][c + 1] << 16), uh[p][c + 2] | (uh[p][c + 3] << 16)};
-                const tz_u32x2 l = {ul[p][c] | (ul[p][c + 1] << 16), ul[p][c + 2] | (ul[p][c + 3] << 16)};
-                *reinterpret_cast<tz_u32x2*>(&sh.u[p][0][c][y0]) = h;
-                *reinterpret_cast<tz_u32x2*>(&sh.u[p][1][c][y0]) = l;
+                if constexpr ( YT == 4 ) {
+                    const tz_u32x2 h = {uh[p][c] | (uh[p][c + 1] << 16), uh[p][c + 2] | (uh[p][c + 3] << 16)};
+                    const tz_u32x2 l = {ul[p][c] | (ul[p][c + 1] << 16), ul[p][c + 2] | (ul[p][c + 3] << 16)};
+                    *reinterpret_cast<tz_u32x2*>(&sh.u[p][0][c][y0]) = h;
+                    *reinterpret_cast<tz_u32x2*>(&sh.u[p][1][c][y0]) = l;
+                }
+                else {
+                    *reinterpret_cast<unsigned*>(&sh.u[p][0][c][y0]) = uh[p][c] | (uh[p][c + 1] << 16);
+                    *reinterpret_cast<unsigned*>(&sh.u[p][1][c][y0]) = ul[p][c] | (ul[p][c + 1] << 16);
+                }
             }
     }
     store_chunk(0, 0);
@@ -228,10 +245,11 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
     };
     // A chunk's K-steps run even ones first, then odd ones (the sum does not depend on the
     // order), so that only one parity's operands are live: within a parity pass, K-step
-    // ks = 8 ck + par + 2 i uses slot (i + a) % 5 for tile a, which holds tile 0's operand of
-    // K-step ks + 2 a.  A pass reads four operands ahead, then during K-step i the one that
-    // tile 3 takes at K-step i + 1 (into the slot tile 0 of K-step i - 1 left).
-    tz_u32x4  aslot[5][3][2];
+    // ks = 8 ck + par + 2 i uses slot (i + a) % NS for tile a, which holds tile 0's operand of
+    // K-step ks + 2 a.  A pass reads kTzTiles operands ahead, then during K-step i the one
+    // that the last tile takes at K-step i + 1 (into the slot tile 0 of K-step i - 1 left).
+    constexpr int NS = kTzTiles + 1; // operand slots of a pass
+    tz_u32x4      aslot[NS][3][2];
     tz_u32x4  bop[2][3][2]; // right operands of the current and the next K-step
     tz_f32x16 acc[kTzTiles][3];
 
@@ -252,11 +270,11 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
         for ( int par = 0; par < 2; par++ ) {
             // every K-step of a chunk runs: those past nks meet zero template rows (the staging
             // zero-fills rows past each template), and their left operands are read at K-steps
-            // clamped to nks + 5, inside the staged column, so no branch splits the pass
+            // clamped to nks + 2 kTzTiles - 3, inside the staged column, so no branch splits the pass
             const int k0 = ck * kTzChunk + par; // first K-step of the pass
-            const int kc = nks + 5;             // largest K-step whose left operand is read
+            const int kc = nks + 2 * kTzTiles - 3; // largest K-step whose left operand is read
 #pragma unroll
-            for ( int sl = 0; sl < 4; sl++ )
+            for ( int sl = 0; sl < kTzTiles; sl++ )
 #pragma unroll
                 for ( int p = 0; p < 3; p++ )
 #pragma unroll
@@ -270,20 +288,21 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
 #pragma unroll
             for ( int i = 0; i < kTzChunk / 2; i++ ) {
                 const int ks = k0 + 2 * i;
-                // the next K-step's operands: tile 3's (tile 0's of K-step ks + 8) and the right ones
+                // the next K-step's operands: the last tile's (tile 0's of K-step ks + 2 kTzTiles)
+                // and the right ones
                 if ( i + 1 < kTzChunk / 2 ) {
 #pragma unroll
                     for ( int p = 0; p < 3; p++ )
 #pragma unroll
                         for ( int h = 0; h < 2; h++ )
-                            aslot[(i + 4) % 5][p][h] = read_a(min(ks + 8, kc), p, h);
+                            aslot[(i + kTzTiles) % NS][p][h] = read_a(min(ks + 2 * kTzTiles, kc), p, h);
 #pragma unroll
                     for ( int p = 0; p < 3; p++ )
 #pragma unroll
                         for ( int h = 0; h < 2; h++ )
                             bop[(i + 1) & 1][p][h] = sh.x[buf][par + 2 * (i + 1)][p][h][g][n];
                 }
-                // hi*hi, hi*lo, lo*hi over the twelve accumulators
+                // hi*hi, hi*lo, lo*hi over the accumulators (3 per tile)
 #pragma unroll
                 for ( int t = 0; t < 3; t++ ) {
                     const int ha = t == 2 ? 1 : 0, hb = t == 1 ? 1 : 0;
@@ -291,7 +310,7 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
                     for ( int p = 0; p < 3; p++ )
 #pragma unroll
                         for ( int a = 0; a < kTzTiles; a++ )
-                            acc[a][p] = tz_mfma(bop[i & 1][p][hb], aslot[(i + a) % 5][p][ha], acc[a][p]);
+                            acc[a][p] = tz_mfma(bop[i & 1][p][hb], aslot[(i + a) % NS][p][ha], acc[a][p]);
                 }
                 // issue order of the K-step: the next K-step's twelve LDS reads two at a time
                 // between the first six matrix instructions (read a full K-step ahead, not
@@ -349,7 +368,7 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
 }
 
 // Grid: x = last workgroups (columns 1 .. last - 1, and workgroup 0 for the pair
-// 0 / last); 256 threads; every workgroup runs all ceil(nitems / 32) template groups.
+// 0 / last); 512 threads; every workgroup runs all ceil(nitems / 32) template groups.
 //   S        frame image in mixed form, column c at S + c * s_stride (Fy = 1024 values)
 //   items    the templates, column-major: a[c * lda + v] element v of column c of template
 //            t (1 <= L_t <= Lmax rows; lda a multiple of 16, at least Lmax), and
@@ -357,7 +376,7 @@ __device__ __forceinline__ void tz_body(TzShared& sh, const XcorrItem* __restric
 //            templates are out_step elements apart (items[t].out = items[0].out + t out_step)
 //   M        output rows (<= 512), Lmax the largest template's rows (<= 512, with
 //            Lmax + M - 1 <= 1024), last the Nyquist column index (ncols - 1)
-__global__ void __launch_bounds__(kTzWaves * 64, 1)
+__global__ void __launch_bounds__(kTzWaves * 64) __attribute__((amdgpu_waves_per_eu(kTzWaves / 4)))
         k_xcorr_toeplitz(const XcorrItem* __restrict__ items, int nitems, long lda, const float2* __restrict__ S,
                          long s_stride, long ldo, long out_step, int M, int Lmax, float scale, int last) {
     __shared__ TzShared sh;
