@@ -94,6 +94,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     B, K = args.batch, args.steps
+    w = N // 2 + 1  # spectrum columns of the micrograph
     only = {x.strip() for x in args.only.split(",") if x.strip()}
 
     def want(section):
@@ -402,7 +403,6 @@ def main() -> int:
         torch.cuda.synchronize()
         del ca, cb
 
-    w = N // 2 + 1
     # compulsory bytes of one step counted once per batch: the image spectrum read once, each template
     # read once and each 512^2 window written once
     step_bytes = 8 * w * N + B * (4 * T * T + 4 * M * M)
