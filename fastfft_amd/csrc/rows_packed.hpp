@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
     const int          t    = threadIdx.x % T;
     const float2*      twm  = multi_twiddles(tw);
     const RowBatchItem it   = load_item(items, blockIdx.y);
-    const int          row0 = (int)blockIdx.x * kCmRows;
+    const int          row0 = xcd_row(blockIdx.x, gridDim.x) * kCmRows; // neighbouring row blocks share one L2
     if ( row0 >= it.rows ) // the whole workgroup, before any barrier
         return;
     const auto* in = as_global(static_cast<const TIn*>(it.in));
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
     const int          t    = threadIdx.x % T;
     const float2*      twm  = multi_twiddles(tw);
     const RowBatchItem it   = load_item(items, blockIdx.y);
-    const int          row0 = (int)blockIdx.x * kCmRows;
+    const int          row0 = xcd_row(blockIdx.x, gridDim.x) * kCmRows; // neighbouring row blocks share one L2
     if ( row0 >= nrows )
         return;
     // ---- the tile: columns k <= P, kCmRows values each, then the pass table
@@ -441,9 +441,10 @@ __global__ void __launch_bounds__(64 * kPackedRows)
     const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int     t   = threadIdx.x % T;
     const float2* twm = multi_twiddles(tw);
-    if ( (int)blockIdx.x * kPackedRows >= nrows )
+    const int rblk = CM ? xcd_row(blockIdx.x, gridDim.x) : (int)blockIdx.x; // (CM: neighbouring row blocks share one L2)
+    if ( rblk * kPackedRows >= nrows )
         return;
-    const int  row  = (int)blockIdx.x * kPackedRows + w;
+    const int  row  = rblk * kPackedRows + w;
     const bool live = row < nrows;
     FFT::template fill_pass_table<NT>(s_twp, twm, threadIdx.x);
     __syncthreads( );
@@ -470,7 +471,7 @@ __global__ void __launch_bounds__(64 * kPackedRows)
         float2             x[E], xp[E];
         if constexpr ( CM ) {
             // the workgroup's rows: kPackedRows consecutive values of every column k <= P
-            const int   r0 = (int)blockIdx.x * kPackedRows;
+            const int   r0 = rblk * kPackedRows;
             const auto* ip = as_global(static_cast<const float2*>(item.in)) + r0;
             __syncthreads( ); // the previous item's tile reads are done
             for ( int k = threadIdx.x; k < P + 1; k += NT ) {
