@@ -193,9 +193,40 @@ static hipError_t go_full_wave(const Stage& s, const StageIO& io, const float2* 
     return hipGetLastError( );
 }
 
+// Full-length real lines into a transposed spectrum (lines adjacent, even element
+// step, 16-byte aligned output): two lines per workgroup as one complex line.
+template <int P, int Q>
+static hipError_t go_full_wave_pair(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const dim3 grid((unsigned)(s.nlines / 2)), block(64 * Q);
+    if ( io.src_rt == RealT::f32 )
+        k_fwd_full_wave_pair<P, Q, float><<<grid, block, 0, st>>>(static_cast<const float*>(io.src), to_device(s.im),
+                                                                  static_cast<float2*>(io.dst), to_device(s.om), s.nout,
+                                                                  s.nlines / 2, io.scale, tw);
+    else
+        k_fwd_full_wave_pair<P, Q, __half><<<grid, block, 0, st>>>(static_cast<const __half*>(io.src), to_device(s.im),
+                                                                   static_cast<float2*>(io.dst), to_device(s.om), s.nout,
+                                                                   s.nlines / 2, io.scale, tw);
+    return hipGetLastError( );
+}
+
+static bool full_wave_pair_covers(const Stage& s, const StageIO& io) {
+    return io.src_real && s.nlines >= 2 && s.nlines % 2 == 0 && s.om.n2 == 1 && s.om.s1 == 1 && s.om.es % 2 == 0 &&
+           s.im.n2 == 1 && reinterpret_cast<std::uintptr_t>(io.dst) % 16 == 0 && s.N >= 1024 &&
+           ! env_int("FASTFFT_NO_PAIR_LINES", 0);
+}
+
 static hipError_t try_full_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( s.L != s.N || io.dst_real || io.dst_ct != CpxT::c64 || s.nout > s.N )
         return hipErrorNotSupported;
+    if ( full_wave_pair_covers(s, io) ) {
+        switch ( s.N ) {
+            case 1024: return go_full_wave_pair<512, 2>(s, io, tw, st);
+            case 2048: return go_full_wave_pair<512, 4>(s, io, tw, st);
+            case 4096: return go_full_wave_pair<512, 8>(s, io, tw, st);
+            case 8192: return go_full_wave_pair<512, 16>(s, io, tw, st);
+            default: break;
+        }
+    }
     switch ( s.N ) {
         case 256: return go_full_wave<256, 1>(s, io, tw, st);
         case 512: return go_full_wave<512, 1>(s, io, tw, st);

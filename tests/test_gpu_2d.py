@@ -214,15 +214,19 @@ FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (25
              ((301, 50), (512, 64)), ((1024, 64), (256, 32))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_packed", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_packed", "no_pair", "generic"])
 @pytest.mark.parametrize("fin,fout", FWD_CASES)
 def test_fwd_random(fin, fout, mode, monkeypatch):
     """Forward transforms against the FP64 oracle with the default kernels (512- and 1024-point real
-    rows on the packed one-wave-per-row kernel), without the packed rows, and generic only."""
+    rows on the packed one-wave-per-row kernel, full-length real rows of 2048 points and up two per
+    workgroup as one complex line), without the packed rows, with one full-length row per workgroup,
+    and generic only."""
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "no_packed":
         monkeypatch.setenv("FASTFFT_NO_PACKED_ROWS", "1")
+    if mode == "no_pair":
+        monkeypatch.setenv("FASTFFT_NO_PAIR_LINES", "1")
     H.log(f"fwd random {fin} -> {fout} mode={mode}")
     img = rand_img(fin[0], fin[1], seed=fin[0] + 7 * fin[1])
     ft = H.make_plan((*fin, 1), (*fout, 1), (*fout, 1), (*fout, 1))
