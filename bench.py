@@ -533,25 +533,25 @@ def main() -> int:
 
         # parity: the FP64 oracle of the same correlation
         spec_h = spec.cpu().numpy()[: 2 * w * N].view(np.complex64).reshape(w, N).astype(np.complex128)
-        want = O.xcorr_2d(W.to_padded(tmpl), T, spec_h, N, N, M, M, -M // 2, -M // 2)
-        wx, wy, _ = O.argmax_2d(want)
+        ref64 = O.xcorr_2d(W.to_padded(tmpl), T, spec_h, N, N, M, M, -M // 2, -M // 2)
+        wx, wy, _ = O.argmax_2d(ref64)
 
         def window(a):
             return a.reshape(M, F.padded_pitch(M))[:, :M].astype(np.float64)
 
         result["parity"] = {
-            "rel_l2_vs_fp64": float(f"{O.rel_l2(window(b_out0), want):.3e}"),
-            "max_abs_vs_fp64": float(f"{O.max_abs(window(b_out0), want):.3e}"),
-            "max_abs_ref": float(f"{np.abs(want).max():.3e}"),
+            "rel_l2_vs_fp64": float(f"{O.rel_l2(window(b_out0), ref64):.3e}"),
+            "max_abs_vs_fp64": float(f"{O.max_abs(window(b_out0), ref64):.3e}"),
+            "max_abs_ref": float(f"{np.abs(ref64).max():.3e}"),
             "argmax_match": [wx, wy] == b_pk,
             "tolerance": 2e-6,
-            "cpu_baseline_rel_l2_vs_fp64": float(f"{O.rel_l2(cb['window'].astype(np.float64), want):.3e}"),
+            "cpu_baseline_rel_l2_vs_fp64": float(f"{O.rel_l2(cb['window'].astype(np.float64), ref64):.3e}"),
         }
         if want("split_bf16"):
-            result["split_bf16"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(s_out0), want):.3e}")
+            result["split_bf16"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(s_out0), ref64):.3e}")
             result["split_bf16"]["tolerance"] = 2e-5
         if want("single"):
-            result["single"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(out.cpu().numpy()), want):.3e}")
+            result["single"]["rel_l2_vs_fp64"] = float(f"{O.rel_l2(window(out.cpu().numpy()), ref64):.3e}")
             result["single"]["argmax_match"] = [wx, wy] == [int(v) for v in peak.cpu().numpy()]
     print(json.dumps(result))
     return 0
