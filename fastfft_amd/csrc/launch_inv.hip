@@ -215,10 +215,29 @@ hipError_t launch_c2r_r2c_rows(const Stage& c, const Stage& f, const float2* src
                                const float2* tw, hipStream_t st) {
     if ( ! c2r_r2c_rows_covers(c, f) )
         return hipErrorNotSupported;
+    // rows per workgroup: neighbouring rows (line stride one element, even element
+    // strides, 16-byte aligned buffers) are moved together by k_c2r_r2c_rows_tile;
+    // FASTFFT_FRAME_ROWS = 1, 2 or 4 sets the count (at most 16 waves per workgroup)
+    const auto paired = [](const HostLineMap& m) { return m.s1 == 1 && m.n2 == 1 && m.es % 2 == 0; };
+    int        rb     = env_int("FASTFFT_FRAME_ROWS", 4);
+    if ( rb != 2 && rb != 4 )
+        rb = 1;
+    if ( rb * (c.N / 1024) > 16 )
+        rb = 2;
+    if ( ! paired(c.im) || ! paired(f.om) || c.nlines % rb != 0 || (reinterpret_cast<uintptr_t>(src) & 15) != 0 ||
+         (reinterpret_cast<uintptr_t>(dst) & 15) != 0 )
+        rb = 1;
     auto go = [&](auto qc) -> hipError_t {
         constexpr int Q = decltype(qc)::value;
-        k_c2r_r2c_rows<Q><<<dim3((unsigned)c.nlines), dim3(64 * Q), 0, st>>>(src, to_device(c.im), c.M, c.o, c.nlines, post,
-                                                                             dst, to_device(f.om), f.nout, pre, tw);
+        if ( rb == 1 )
+            k_c2r_r2c_rows<Q><<<dim3((unsigned)c.nlines), dim3(64 * Q), 0, st>>>(
+                    src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
+        else if ( rb == 2 )
+            k_c2r_r2c_rows_tile<Q, 2><<<dim3((unsigned)(c.nlines / 2)), dim3(64 * Q * 2), 0, st>>>(
+                    src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
+        else if constexpr ( Q <= 4 )
+            k_c2r_r2c_rows_tile<Q, 4><<<dim3((unsigned)(c.nlines / 4)), dim3(64 * Q * 4), 0, st>>>(
+                    src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
         return hipGetLastError( );
     };
     return c.N == 2048 ? go(std::integral_constant<int, 2>{ })

@@ -801,4 +801,135 @@ __global__ void __launch_bounds__(64 * Q)
         store_cpx(op, (long)P * om2.es, make_float2(v[0].x - v[0].y, 0.f));
 }
 
+// k_c2r_r2c_rows on RB neighbouring rows per workgroup (RB Q waves, Q per row).
+// Rows y0 .. y0 + RB - 1 of the input and of the output map lie next to each
+// other (line stride one element, element stride even), so element k of the
+// RB rows is one contiguous segment of 8 RB bytes, moved as RB / 2 16-byte
+// accesses instead of RB scattered 8-byte ones (the frame's rows are columns
+// of the y-pass output, 8 KB apart per element).  Each row is computed as in
+// k_c2r_r2c_rows by its own Q waves; the RB rows' spectrum values are gathered
+// in LDS and the whole workgroup stores them.
+template <int Q, int RB>
+__global__ void __launch_bounds__(64 * Q * RB)
+        k_c2r_r2c_rows_tile(const float2* __restrict__ in, LineMap im, int M, int o, int nlines, float post_scale,
+                            float2* __restrict__ out, LineMap om2, int nout, float pre_scale,
+                            const float2* __restrict__ tw) {
+    static_assert(RB == 2 || RB == 4, "two or four rows per workgroup");
+    constexpr int P = 512, T = 64, E = P / T, PP = P * Q, N = 2 * PP, NT = T * Q, N2 = 2 * P, NB = NT * RB;
+    constexpr int RP = RB / 2; // 16-byte row pairs per element
+    using FFT       = WaveFFT<P>;
+    constexpr int LDS = (Q * FFT::SLOTS > PP + 1 + PP / 32 ? Q * FFT::SLOTS : PP + 1 + PP / 32);
+    constexpr int ROW = P + 2; // s_row pitch: the real row as P complex values, later the P + 1 spectrum values
+    __shared__ float2 lds[RB][LDS];
+    __shared__ float2 s_twp[P];
+    __shared__ float2 s_row[RB][ROW];
+    const int     w   = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int     r   = w / Q; // the workgroup's row of this wave
+    const int     q   = w % Q; // its sub-transform
+    const int     t   = threadIdx.x % T;
+    const int     tr  = threadIdx.x % NT; // thread index within the row
+    const int     y0  = xcd_row(blockIdx.x, nlines / RB) * RB;
+    const float2* twm = multi_twiddles(tw);
+    auto          pad = [](int k) { return k + (k >> 5); };
+    {
+        float2 ptv[FFT::template pass_table_rows<NB>];
+        FFT::template load_pass_table<NB>(ptv, twm, threadIdx.x);
+        constexpr int R  = ((PP + 1) * RP + NB - 1) / NB;
+        const auto*   lp = as_global(in) + im.base(y0);
+        float4        xr[R];
+#pragma unroll
+        for ( int rr = 0; rr < R; rr++ ) {
+            const int g = threadIdx.x + NB * rr;
+            const int k = g / RP, p = g % RP;
+            xr[rr]      = k <= PP ? load_cpx_pair(lp, ((long)k * im.es + 2 * p) >> 1) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        FFT::template store_pass_table<NB>(s_twp, ptv, threadIdx.x);
+#pragma unroll
+        for ( int rr = 0; rr < R; rr++ ) {
+            const int g = threadIdx.x + NB * rr;
+            const int k = g / RP, p = g % RP;
+            if ( k <= PP ) {
+                lds[2 * p][pad(k)]     = make_float2(xr[rr].x, xr[rr].y);
+                lds[2 * p + 1][pad(k)] = make_float2(xr[rr].z, xr[rr].w);
+            }
+        }
+    }
+    __syncthreads( );
+    float2* rl = lds[r];
+    FFT     fft;
+    fft.init_pt(t, s_twp);
+    float2 v[E];
+#pragma unroll
+    for ( int i = 0; i < E; i++ ) {
+        const int k  = Q * (t + T * i) + q;
+        float2    xk = rl[pad(k)], xp = rl[pad(PP - k)];
+        if ( k == 0 ) {
+            xk.y = 0.f;
+            xp.y = 0.f;
+        }
+        const float2 e = make_float2(xk.x + xp.x, xk.y - xp.y);     // X + conj Xp
+        const float2 d = make_float2(xk.x - xp.x, xk.y + xp.y);     // X - conj Xp
+        const float2 c = cmulc(d, load_cpx(as_global(twm), N + k)); // W_N^{-k} d
+        v[i]           = make_float2(e.x - c.y, e.y + c.x);         // e + i c
+    }
+    __syncthreads( );
+    float2* wl = rl + q * FFT::SLOTS;
+    fft.template run<+1>(v, wl, t);
+#pragma unroll
+    for ( int i = 0; i < E; i++ )
+        wl[t + T * i] = v[i];
+    __syncthreads( );
+    // ---- the window row, zero past M, into s_row[r]
+    const int oN = ((o % N) + N) % N;
+    float*    xw = reinterpret_cast<float*>(s_row[r]);
+    for ( int j = tr; j < N2; j += NT ) {
+        float y = 0.f;
+        if ( j < M ) {
+            const int    n  = (oN + j) & (N - 1);
+            const int    m  = n >> 1;
+            const int    mm = m & (P - 1);
+            const float2 wm = load_cpx(as_global(twm), PP + m);
+            float2       z  = rl[(Q - 1) * FFT::SLOTS + mm];
+#pragma unroll
+            for ( int qq = Q - 2; qq >= 0; qq-- )
+                z = cadd(cmulc(z, wm), rl[qq * FFT::SLOTS + mm]);
+            y = ((n & 1) ? z.y : z.x) * post_scale;
+        }
+        xw[j] = y;
+    }
+    __syncthreads( ); // U_q are read; wave 0 of each row reuses the row's region for the R2C
+    if ( q == 0 ) {
+        const float2 wt = load_cpx(as_global(twm), N2 + t); // W_N2^t
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const float2 a = s_row[r][t + T * i];
+            v[i]           = make_float2(a.x * pre_scale, a.y * pre_scale);
+        }
+        fft.template run<-1>(v, rl, t); // v[i] = Z[t + 64 i]
+#pragma unroll
+        for ( int i = 0; i < E; i++ )
+            rl[FFT::Layout::slot(t + T * i)] = v[i];
+        wave_lds_fence( );
+        // the spectrum X[0 .. P] of the row into s_row[r] (the real row was read into v)
+#pragma unroll
+        for ( int i = 0; i < E; i++ ) {
+            const int    k  = t + T * i;
+            const float2 zp = rl[FFT::Layout::slot((P - k) & (P - 1))];
+            const float2 a  = make_float2(0.5f * (v[i].x + zp.x), 0.5f * (v[i].y - zp.y)); // (Z + conj Zp) / 2
+            const float2 b  = make_float2(0.5f * (v[i].x - zp.x), 0.5f * (v[i].y + zp.y)); // (Z - conj Zp) / 2
+            const float2 c  = cmul(cmul(twm[N2 + T * i], wt), b);                          // W_N2^k b
+            s_row[r][k]     = make_float2(a.x + c.y, a.y - c.x);                            // a - i c
+        }
+        if ( t == 0 )
+            s_row[r][P] = make_float2(v[0].x - v[0].y, 0.f);
+    }
+    __syncthreads( );
+    // ---- the RB rows' spectra, element k of the rows as RB / 2 16-byte stores
+    auto* op = as_global(out) + om2.base(y0);
+    for ( int g = threadIdx.x; g < nout * RP; g += NB ) {
+        const int k = g / RP, p = g % RP;
+        store_cpx_pair(op, ((long)k * om2.es + 2 * p) >> 1, s_row[2 * p][k], s_row[2 * p + 1][k]);
+    }
+}
+
 } // namespace fastfft_amd

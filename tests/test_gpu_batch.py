@@ -60,8 +60,8 @@ FRAMES = {((4096, 4096), (512, 512)): "1024x1024", ((2048, 2048), (150, 200)): "
           ((2048, 2048), (512, 400)): "1024x1024", ((8192, 1024), (512, 64)): "1024x512"}
 
 
-@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major", "full_frame",
-                                  "per_template"])
+@pytest.mark.parametrize("mode", ["tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major", "frame_rows1",
+                                  "frame_rows2", "full_frame", "per_template"])
 @pytest.mark.parametrize("sizes,n,nout,origin", BATCHES)
 def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeypatch):
     """Batched kernels (chunk size from the tuner, or 2 to force several chunks, or the two-wave row
@@ -80,10 +80,13 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         monkeypatch.setenv("FASTFFT_NO_SUBFRAME", "1")
     if mode == "row_major":
         monkeypatch.setenv("FASTFFT_BATCH_ROW_MAJOR", "1")
+    if mode.startswith("frame_rows"):
+        monkeypatch.setenv("FASTFFT_FRAME_ROWS", mode[-1])
     H.log(f"batch {sizes} in {n} -> {nout} at {origin} mode={mode}")
     torch, ft, spec, img, tmpls, hosts, outs, peaks, vals = _setup(sizes, n, nout, origin, seed=len(sizes))
     form = ft.Describe("batch").splitlines()[0]
-    frame = FRAMES.get((n, nout)) if mode in ("tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major") else None
+    framed = ("tuned", "chunk2", "no_packed", "no_frame_fusion", "row_major", "frame_rows1", "frame_rows2")
+    frame = FRAMES.get((n, nout)) if mode in framed else None
     assert f"frame={frame or 'full'}" in form, form
     if frame:
         # the frame's template and output slots: column-major where the tiled row kernels cover the
@@ -105,19 +108,22 @@ def test_batch_matches_single_correlations(sizes, n, nout, origin, mode, monkeyp
         assert tuple(pk[i]) == (rx, ry) and vv[i] == np.float32(rv), i
         # the FP64 oracle of the full N-frame pipeline: every small batch, and the batches on a reduced
         # frame in the modes that run the frame (the fused frame rows at Q = 2, 4 and 8 included)
-        if n[0] * n[1] <= 256 * 128 or (frame and mode in ("tuned", "no_packed", "no_frame_fusion", "row_major")):
+        if n[0] * n[1] <= 256 * 128 or (frame and mode != "chunk2"):
             want = O.xcorr_2d(H.padded_real(hosts[i], x), x, spec, n[0], n[1], nout[0], nout[1], *origin)
             assert O.rel_l2(got, want) < TOL, i
             wx, wy, _ = O.argmax_2d(want)
             assert tuple(pk[i]) == (wx, wy), i
 
 
+@pytest.mark.parametrize("rows", [1, 2, 4])
 @pytest.mark.parametrize("case", [0, 4, 5])
-def test_frame_fusion_matches_two_kernels(case, monkeypatch):
-    """The frame's fused C2R + R2C rows (Q = 4, 2 and 8 for the 4096-, 2048- and 8192-point images) give the
-    outputs and peaks of the same batch with the two kernels run one after the other, to FP32 rounding."""
+def test_frame_fusion_matches_two_kernels(case, rows, monkeypatch):
+    """The frame's fused C2R + R2C rows (Q = 4, 2 and 8 for the 4096-, 2048- and 8192-point images; one, two
+    or four rows per workgroup, at most two at Q = 8) give the outputs and peaks of the same batch with the
+    two kernels run one after the other, to FP32 rounding."""
     sizes, n, nout, origin = BATCHES[case]
-    H.log(f"frame fusion against two kernels, {sizes} in {n} -> {nout}")
+    H.log(f"frame fusion against two kernels, {sizes} in {n} -> {nout}, {rows} rows per workgroup")
+    monkeypatch.setenv("FASTFFT_FRAME_ROWS", str(rows))
     res = {}
     for fused in (True, False):
         if not fused:
