@@ -1,7 +1,8 @@
 """Split-BF16 matrix-core form of the batched correlation (fastfft_set_precision).
 
-The correlation pass of each 1024 x 1024 frame runs its DFT-32 stages as hi + lo BF16 matrix products
-with FP32 accumulation; every other pass stays FP32.  Outputs are compared with the FP64 oracle of the
+The correlation pass of each 1024 x 1024 frame runs as Toeplitz matrix products per column (frame
+column against 32 templates at a time, Gauss's three-product complex form) on hi + lo BF16 operands with
+FP32 accumulation; every other pass stays FP32.  Outputs are compared with the FP64 oracle of the
 full 4096 x 4096 pipeline (relative L2 within 2e-5; the FP32 form of the same batch within 2e-6), and every peak must equal the FP32 path's, the
 oracle's and the planted lag.  The FP16 I/O case has the shape of 512 x 512 FP16 templates (radius-220
 masks) against a 4096 x 4096 FP16 micrograph; there FP16 storage sets the error bound (2e-3, as for the
