@@ -91,7 +91,7 @@ template <int P, typename TIn>
 static hipError_t go_r2c_packed_cm(const Stage& s, const RowBatchItem* items, int n, int max_rows, float pre,
                                    const float2* tw, hipStream_t st) {
     const dim3 grid((unsigned)((max_rows + kCmRows - 1) / kCmRows), (unsigned)n);
-    k_r2c_rows_packed_cm<P, TIn><<<grid, dim3(64 * kPackedRows), 0, st>>>(s.om.es, s.nout, pre, tw, items);
+    k_r2c_rows_packed_cm<P, TIn><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.om.es, s.nout, pre, tw, items);
     return hipGetLastError( );
 }
 

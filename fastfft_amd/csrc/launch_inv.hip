@@ -329,7 +329,7 @@ template <int P, typename TOut>
 static hipError_t go_c2r_packed_cm(const Stage& s, const RowBatchItem* items, int n, float post, const float2* tw,
                                    hipStream_t st) {
     const dim3 grid((unsigned)((s.nlines + kCmRows - 1) / kCmRows), (unsigned)n);
-    k_c2r_rows_packed_cm<P, TOut><<<grid, dim3(64 * kPackedRows), 0, st>>>(s.im.es, to_device(s.om), s.M, s.o, s.nlines,
+    k_c2r_rows_packed_cm<P, TOut><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.im.es, to_device(s.om), s.M, s.o, s.nlines,
                                                                           post, tw, items);
     return hipGetLastError( );
 }

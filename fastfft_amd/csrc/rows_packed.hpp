@@ -199,15 +199,16 @@ __global__ void __launch_bounds__(64 * kPackedRows)
 // tile doubles as the exchange region of the wave that transforms it, so the tile
 // costs no LDS beyond the waves' own regions.
 constexpr int kCmRows = 8;
+constexpr int kCmWaves = 8; // waves per workgroup of the column-major row kernels (one row each)
 
 // k_r2c_rows_packed for batches with column-major output (ldc: elements between
 // columns, even).  Grid: ceil(max rows / kCmRows) x templates.  Rows past the
 // template's row count are stored as zeros up to the end of the workgroup's rows.
 template <int P, typename TIn>
-__global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * kCmWaves)
         k_r2c_rows_packed_cm(long ldc, int nout, float pre_scale, const float2* __restrict__ tw,
                              const RowBatchItem* __restrict__ items) {
-    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows, RPW = kCmRows / kPackedRows;
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kCmWaves, RPW = kCmRows / kCmWaves;
     using FFT       = WaveFFT<P>;
     // tile row pitch: a row holds X[0 .. P] and the wave's exchange region, and rows start
     // 16 bytes apart modulo 128 (the column accesses of kCmRows rows hit distinct banks)
@@ -224,14 +225,14 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
         return;
     const auto* in = as_global(static_cast<const TIn*>(it.in));
     // ---- every load of the thread issued before its LDS writes: the pass table and the
-    // wave's rows row0 + w + kPackedRows r
+    // wave's rows row0 + w + kCmWaves r
     float2 ptv[FFT::template pass_table_rows<NT>];
     FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
     const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
     float2       v[RPW][E];
 #pragma unroll
     for ( int r = 0; r < RPW; r++ ) {
-        const int   row = row0 + w + kPackedRows * r;
+        const int   row = row0 + w + kCmWaves * r;
         const auto* rp  = in + (long)min(row, it.rows - 1) * it.in_ld;
 #pragma unroll
         for ( int i = 0; i < E; i++ ) {
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
     fft.init_pt(t, s_twp);
 #pragma unroll
     for ( int r = 0; r < RPW; r++ ) {
-        const int  rr   = w + kPackedRows * r; // row within the tile, and the wave's region
+        const int  rr   = w + kCmWaves * r; // row within the tile, and the wave's region
         const bool live = row0 + rr < it.rows;
         float2*    wl   = tile + rr * TP;
 #pragma unroll
@@ -288,10 +289,10 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
 // columns, even; the input column holds at least ceil(nrows / kCmRows) kCmRows
 // values).  Grid: ceil(nrows / kCmRows) x templates.
 template <int P, typename TOut>
-__global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * kCmWaves)
         k_c2r_rows_packed_cm(long ldc, LineMap om, int M, int o, int nrows, float post_scale, const float2* __restrict__ tw,
                              const RowBatchItem* __restrict__ items) {
-    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kPackedRows, RPW = kCmRows / kPackedRows;
+    constexpr int T = 64, E = P / T, N = 2 * P, NT = T * kCmWaves, RPW = kCmRows / kCmWaves;
     using FFT       = WaveFFT<P>;
     // tile row pitch: a row holds X[0 .. P] and the wave's exchange region, and rows start
     // 16 bytes apart modulo 128 (the column accesses of kCmRows rows hit distinct banks)
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(64 * kPackedRows) __attribute__((amdgpu_waves_
     const int oN = ((o % N) + N) % N;
 #pragma unroll
     for ( int r = 0; r < RPW; r++ ) {
-        const int rr  = w + kPackedRows * r;
+        const int rr  = w + kCmWaves * r;
         const int row = row0 + rr;
         if ( row >= nrows ) // wave-uniform; no barrier follows
             break;
