@@ -147,6 +147,14 @@ hipError_t launch_stats_reset(const BatchStats& bs, long n, hipStream_t st) {
 static hipError_t try_c2r_packed(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( ! c2r_packed_covers(s) || io.src_real || io.src_ct != CpxT::c64 || ! io.dst_real )
         return hipErrorNotSupported;
+    // rows whose elements lie far apart (a 3-D inverse's x-pass, a 2-D InvFFT's 512-point rows):
+    // the tiled line kernel reads 16 neighbouring rows per 128-byte access, where one wave per
+    // row would gather 8 bytes per element
+    if ( s.im.es != 1 && ! io.amax && ! env_int("FASTFFT_NO_TILE_LINES", 0) ) {
+        const hipError_t e = launch_tile_lines(s, io, tw, st);
+        if ( e != hipErrorNotSupported )
+            return e;
+    }
     const bool keys = io.amax && s.nlines <= kArgmaxPartials;
     hipError_t e    = launch_c2r_packed(s, io.src, io.dst, io.dst_rt, io.scale, tw, keys ? io.amax->keys : nullptr,
                                         nullptr, 1, st);

@@ -11,6 +11,12 @@ namespace fastfft_amd {
 
 static bool is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// A set, non-zero environment switch (read when a plan's stage lists are built).
+static bool env_flag_planner(const char* name) {
+    const char* v = std::getenv(name);
+    return v && *v && std::atoi(v) != 0;
+}
+
 static int next_pow2(int n) {
     int p = 1;
     while ( p < n )
@@ -307,26 +313,53 @@ static PlanError inv_stages_3d(const PlanState& p, std::vector<Stage>& st, Buf s
     }
     const bool dec = p.inv_change == SizeChange::decrease;
     const int  w   = in.w;
-    // z-pass on the w*in.y lines -> T1[jz][kx][ky] (out.z planes)
-    Stage s1    = make_stage(StageKind::inv_cpx, dec ? "c2c_inv_decrease_XZ" : "c2c_inv_none_XZ", in.z, 0, out.z,
-                             axis_origin(p, 2, in.z, out.z), 0, w * in.y);
-    s1.src      = src;
-    s1.src_elem = src_elem;
-    s1.im       = lm(1, 0, (long)w * in.y, 1);
-    s1.dst      = Buf::ws_a;
-    s1.dst_elem = Elem::cpx_f32;
-    s1.om       = lm(1, 0, (long)w * in.y, 1);
-    st.push_back(s1);
-    // y-pass on the out.z*w lines -> T2[jz][kx][jy]
-    Stage s2    = make_stage(StageKind::inv_cpx, dec ? "c2c_inv_decrease_XYZ" : "c2c_inv_none_XYZ", in.y, 0, out.y,
-                             axis_origin(p, 1, in.y, out.y), 0, out.z * w);
-    s2.src      = Buf::ws_a;
-    s2.src_elem = Elem::cpx_f32;
-    s2.im       = lm(in.y, 0, 1, 1);
-    s2.dst      = Buf::ws_b;
-    s2.dst_elem = Elem::cpx_f32;
-    s2.om       = lm(out.y, 0, 1, 1);
-    st.push_back(s2);
+    if ( out.y < in.y && ! env_flag_planner("FASTFFT_INV3D_Z_FIRST") ) {
+        // the y window is trimmed: the y-pass first, on the contiguous ky lines of the whole
+        // input, so that the strided z-pass reads out.y / in.y of the input's volume (both
+        // orders move the same bytes; this one moves the most of them with unit stride)
+        // y-pass on the in.z*w lines -> T1[kz][kx][jy]
+        Stage s1    = make_stage(StageKind::inv_cpx, "c2c_inv_decrease_XYZ", in.y, 0, out.y, axis_origin(p, 1, in.y, out.y),
+                                 0, in.z * w);
+        s1.src      = src;
+        s1.src_elem = src_elem;
+        s1.im       = lm(in.y, 0, 1, 1);
+        s1.dst      = Buf::ws_a;
+        s1.dst_elem = Elem::cpx_f32;
+        s1.om       = lm(out.y, 0, 1, 1);
+        st.push_back(s1);
+        // z-pass on the w*out.y lines -> T2[jz][kx][jy]
+        Stage s2    = make_stage(StageKind::inv_cpx, dec && out.z < in.z ? "c2c_inv_decrease_XZ" : "c2c_inv_none_XZ", in.z, 0,
+                                 out.z, axis_origin(p, 2, in.z, out.z), 0, w * out.y);
+        s2.src      = Buf::ws_a;
+        s2.src_elem = Elem::cpx_f32;
+        s2.im       = lm(1, 0, (long)w * out.y, 1);
+        s2.dst      = Buf::ws_b;
+        s2.dst_elem = Elem::cpx_f32;
+        s2.om       = lm(1, 0, (long)w * out.y, 1);
+        st.push_back(s2);
+    }
+    else {
+        // z-pass on the w*in.y lines -> T1[jz][kx][ky] (out.z planes)
+        Stage s1    = make_stage(StageKind::inv_cpx, dec ? "c2c_inv_decrease_XZ" : "c2c_inv_none_XZ", in.z, 0, out.z,
+                                 axis_origin(p, 2, in.z, out.z), 0, w * in.y);
+        s1.src      = src;
+        s1.src_elem = src_elem;
+        s1.im       = lm(1, 0, (long)w * in.y, 1);
+        s1.dst      = Buf::ws_a;
+        s1.dst_elem = Elem::cpx_f32;
+        s1.om       = lm(1, 0, (long)w * in.y, 1);
+        st.push_back(s1);
+        // y-pass on the out.z*w lines -> T2[jz][kx][jy]
+        Stage s2    = make_stage(StageKind::inv_cpx, dec ? "c2c_inv_decrease_XYZ" : "c2c_inv_none_XYZ", in.y, 0, out.y,
+                                 axis_origin(p, 1, in.y, out.y), 0, out.z * w);
+        s2.src      = Buf::ws_a;
+        s2.src_elem = Elem::cpx_f32;
+        s2.im       = lm(in.y, 0, 1, 1);
+        s2.dst      = Buf::ws_b;
+        s2.dst_elem = Elem::cpx_f32;
+        s2.om       = lm(out.y, 0, 1, 1);
+        st.push_back(s2);
+    }
     // x-pass C2R on the out.z*out.y rows -> out[jz][jy][pitch]
     const int pitch = 2 * out.w;
     Stage     s3    = make_stage(StageKind::inv_c2r, dec ? "c2r_decrease" : "c2r_none", in.x, 0, out.x,

@@ -45,10 +45,15 @@ def test_fwd_3d_vs_oracle(vin, n):
 
 @pytest.mark.parametrize("n,nout,origin", [((32, 32, 16), (16, 8, 12), (-8, -4, -6)), ((16, 16, 16), (16, 16, 16), (0, 0, 0)),
                                             ((64, 32, 32), (20, 32, 8), (0, 0, 3)), ((128, 128, 128), (32, 32, 32), (-16, -16, -16))])
-def test_inv_3d_vs_oracle(n, nout, origin):
+@pytest.mark.parametrize("order", ["default", "z_first"])
+def test_inv_3d_vs_oracle(n, nout, origin, order, monkeypatch):
+    """Trimmed 3-D inverses against the FP64 oracle: by default a trimmed y window puts the y-pass
+    first (unit-stride lines of the whole input), FASTFFT_INV3D_Z_FIRST=1 the z-pass."""
     from fastfft_amd import workloads as W
 
-    H.log(f"3d inv {n} -> {nout} at {origin}")
+    if order == "z_first":
+        monkeypatch.setenv("FASTFFT_INV3D_Z_FIRST", "1")
+    H.log(f"3d inv {n} -> {nout} at {origin} order={order}")
     rng = np.random.default_rng(n[0] + nout[0])
     w = n[0] // 2 + 1
     spec = (rng.standard_normal((n[2], w, n[1])) + 1j * rng.standard_normal((n[2], w, n[1]))).astype(np.complex64)
@@ -177,11 +182,14 @@ def test_fwd_3d_tiled_z_pass(vin, n, mode, monkeypatch):
     assert O.rel_l2(got, want) < TOL
 
 
-@pytest.mark.parametrize("mode", ["default", "no_tile"])
+@pytest.mark.parametrize("mode", ["default", "no_tile", "z_first"])
 def test_inv_3d_tiled_z_pass_window(mode, monkeypatch):
-    """A trimmed inverse whose z-pass keeps 100 of 512 lags at an origin off the 64-lane grid."""
+    """A trimmed inverse whose z-pass keeps 100 of 512 lags at an origin off the 64-lane grid (after the
+    y-pass by default, before it with FASTFFT_INV3D_Z_FIRST=1)."""
     if mode == "no_tile":
         monkeypatch.setenv("FASTFFT_NO_TILE_LINES", "1")
+    if mode == "z_first":
+        monkeypatch.setenv("FASTFFT_INV3D_Z_FIRST", "1")
     n, nout, origin = (128, 64, 512), (100, 60, 100), (-50, -30, -37)
     H.log(f"3d inv {n} -> {nout} at {origin}, z-pass mode={mode}")
     rng = np.random.default_rng(77)
