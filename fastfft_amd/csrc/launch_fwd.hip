@@ -252,7 +252,10 @@ hipError_t launch_fwd(const Stage& s, const StageIO& io, const float2* tw, hipSt
         if ( e != hipErrorNotSupported )
             return e;
     }
-    if ( s.kind == StageKind::fwd_cpx && ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
+    // C2C passes, and real rows whose spectra are stored transposed (a 3-D x-pass): 16 lines per
+    // workgroup, transposed through LDS into 128-byte segments
+    if ( (s.kind == StageKind::fwd_cpx || (s.kind == StageKind::fwd_real && s.om.es != 1)) &&
+         ! env_int("FASTFFT_NO_WAVE_ROWS", 0) ) {
         hipError_t e = launch_tile_lines(s, io, tw, st);
         if ( e != hipErrorNotSupported )
             return e;

@@ -27,6 +27,8 @@ bool tile_lines_cover(const Stage& s) {
         return false;
     if ( s.kind == StageKind::inv_c2r )
         return tile_map(s.im) && row_map(s.om);
+    if ( s.kind == StageKind::fwd_real ) // real rows in, transposed spectrum rows out
+        return row_map(s.im) && tile_map(s.om) && s.nout <= s.N;
     if ( s.kind != StageKind::fwd_cpx && s.kind != StageKind::inv_cpx )
         return false;
     return (tile_map(s.im) && tile_map(s.om)) || (row_map(s.im) && row_map(s.om));
@@ -35,12 +37,19 @@ bool tile_lines_cover(const Stage& s) {
 hipError_t launch_tile_lines(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if ( env_int("FASTFFT_NO_TILE_LINES", 0) || ! tile_lines_cover(s) )
         return hipErrorNotSupported;
-    const bool fwd = s.kind == StageKind::fwd_cpx, c2r = s.kind == StageKind::inv_c2r;
+    const bool r2c = s.kind == StageKind::fwd_real;
+    const bool fwd = s.kind == StageKind::fwd_cpx || r2c, c2r = s.kind == StageKind::inv_c2r;
     const int  M = fwd ? s.nout : s.M, o = fwd ? 0 : s.o;
-    if ( M < 1 || M > s.N || io.src_real )
+    if ( M < 1 || M > s.N || io.src_real != r2c )
         return hipErrorNotSupported;
     auto go = [&](auto nc) -> hipError_t {
         constexpr int N = decltype(nc)::value;
+        if ( r2c ) {
+            if ( io.dst_real || io.dst_ct != CpxT::c64 )
+                return hipErrorNotSupported;
+            return io.src_rt == RealT::f32 ? go_tile<N, -1, float, false, float2, false, true>(s, io, tw, st, M, o)
+                                           : go_tile<N, -1, __half, false, float2, false, true>(s, io, tw, st, M, o);
+        }
         if ( c2r ) {
             if ( ! io.dst_real || io.src_ct != CpxT::c64 )
                 return hipErrorNotSupported;

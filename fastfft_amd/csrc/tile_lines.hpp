@@ -38,6 +38,9 @@ __global__ void __launch_bounds__(512)
                      int nlines, float scale, const float2* __restrict__ tw) {
     constexpr int  T = 64, E = N / T, LT = kTileLines, NW = 8, PITCH = N + 2, RPT = N / (512 / LT), H = N / 2;
     constexpr bool REAL_OUT = ! std::is_same<TOut, float2>::value;
+    // real input lines (the R2C x-pass of a 3-D forward, as a complex transform of x + 0 i)
+    constexpr bool REAL_IN = std::is_same<TIn, float>::value || std::is_same<TIn, __half>::value;
+    static_assert(! REAL_IN || (! TILE_IN && ! HERM && DIR < 0), "real input lines are read directly, forward");
     using FFT               = WaveFFT<N>;
     static_assert(FFT::SLOTS <= PITCH, "a tile row must hold the exchange buffer of its transform");
     static_assert(! (REAL_OUT && TILE_OUT), "real outputs are stored by row");
@@ -99,6 +102,9 @@ __global__ void __launch_bounds__(512)
                     const bool   lo = n <= H;
                     const float2 z  = load_cpx(lp, (long)(lo ? n : N - n) * im.es);
                     v[i]            = lo ? z : make_float2(z.x, -z.y);
+                }
+                else if constexpr ( REAL_IN ) {
+                    v[i] = make_float2(n < kin ? load_real(lp, (long)n * im.es) : 0.f, 0.f);
                 }
                 else {
                     v[i] = n < kin ? load_cpx(lp, (long)n * im.es) : make_float2(0.f, 0.f);
