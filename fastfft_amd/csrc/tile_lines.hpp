@@ -3,9 +3,11 @@
 // C2C_INCREASE / C2C_NONE_XYZ / C2C_DECREASE / C2R_DECREASE with the XZ/XYZ
 // permutes, src/fastfft/FastFFT.cu and include/detail/memory_addressing.h:25-104).
 //
-// A workgroup of 8 wavefronts takes kTileLines lines; each wave transforms two
-// of them in registers (one P = N point Stockham transform, no sub-transform
-// split), so the twiddle set-up is shared by 16 lines.
+// A workgroup of 8 wavefronts takes kTileLines lines (tile modes), each wave
+// transforming two of them in registers (one P = N point Stockham transform, no
+// sub-transform split), so the twiddle set-up is shared by 16 lines; with
+// neither TILE_IN nor TILE_OUT (direct lines) it takes 8 lines, one per wave,
+// so that a wave's chain of loads, transform and stores is not run twice.
 //   TILE_IN:  the lines lie next to each other and their elements far apart (a
 //             z-pass): element rows of the tile (16 consecutive values, one
 //             128-byte access) are loaded into LDS, all of a thread's loads
@@ -30,13 +32,18 @@ namespace fastfft_amd {
 
 constexpr int kTileLines = 16;
 
-// Grid: nlines / kTileLines workgroups of 512 threads.  TILE_IN / TILE_OUT need
+// Lines per workgroup: kTileLines in the tile modes, one per wave for direct lines.
+template <bool TILE_IN, bool TILE_OUT>
+constexpr int tile_workgroup_lines = TILE_IN || TILE_OUT ? kTileLines : 8;
+
+// Grid: nlines / tile_workgroup_lines workgroups of 512 threads.  TILE_IN / TILE_OUT need
 // lines l .. l + 15 of a tile at base(l) + 0 .. 15 (unit stride between lines).
 template <int N, int DIR, typename TIn, bool HERM, typename TOut, bool TILE_IN, bool TILE_OUT>
 __global__ void __launch_bounds__(512)
         k_tile_lines(const TIn* __restrict__ in, LineMap im, int L, TOut* __restrict__ out, LineMap om, int M, int o,
                      int nlines, float scale, const float2* __restrict__ tw) {
-    constexpr int  T = 64, E = N / T, LT = kTileLines, NW = 8, PITCH = N + 2, RPT = N / (512 / LT), H = N / 2;
+    constexpr int  T = 64, E = N / T, LT = tile_workgroup_lines<TILE_IN, TILE_OUT>, NW = 8, PITCH = N + 2, H = N / 2;
+    constexpr int  RPT = N / (512 / LT);
     constexpr bool REAL_OUT = ! std::is_same<TOut, float2>::value;
     // real input lines (the R2C x-pass of a 3-D forward, as a complex transform of x + 0 i)
     constexpr bool REAL_IN = std::is_same<TIn, float>::value || std::is_same<TIn, __half>::value;
