@@ -34,7 +34,9 @@ def _fwd_gpu(v, n):
 
 
 @pytest.mark.parametrize("vin,n", [((16, 12, 10), (32, 32, 16)), ((8, 8, 8), (8, 8, 8)), ((20, 9, 30), (32, 16, 64)),
-                                   ((64, 64, 64), (128, 128, 128))])
+                                   ((64, 64, 64), (128, 128, 128)),
+                                   # 256-point x-passes: real rows through the tile kernel, spectra stored transposed
+                                   ((40, 20, 12), (256, 32, 16)), ((200, 16, 3), (256, 16, 8))])
 def test_fwd_3d_vs_oracle(vin, n):
     H.log(f"3d fwd {vin} -> {n}")
     v = _vol(vin[::-1], seed=sum(vin))

@@ -161,6 +161,23 @@ def _spec_host_3d(buf, n):
     return (a[0::2] + 1j * a[1::2]).reshape(n[2], n[0] // 2 + 1, n[1])
 
 
+def test_half_3d_fwd_256_rows():
+    """<float, __half, __half2, 3> forward of an FP16 volume whose 256-point x-pass takes FP16 rows through
+    the tile kernel (spectra stored transposed), against the oracle on the same FP16-rounded input."""
+    vin, n = (40, 20, 12), (256, 32, 16)
+    H.log(f"<float, __half, __half2, 3> fwd {vin} -> {n}")
+    rng = np.random.default_rng(41)
+    v = rng.standard_normal(vin[::-1]).astype(np.float16)
+    ft = H.make_plan(vin, n, n, n, input_type="f16", image_type="c32h", rank=3)
+    a = H.to_dev(H.padded_real(v, vin[0], np.float16).reshape(-1))
+    out = H.dev_zeros(ft.ReturnFwdOutputMemorySize(), "f16")
+    pre = 1.0 / np.sqrt(vin[0] * vin[1] * vin[2])
+    ft.FwdFFT(a, out, pre_op=H.F.Scale(pre))
+    got = _spec_host_3d(out, n)
+    want = O.fwd_3d(H.padded_real(v.astype(np.float32), vin[0]), vin[0], *n) * pre
+    assert O.rel_l2(got, want) < TOL_F16
+
+
 @pytest.mark.parametrize("tin,n,nout,origin", [((8, 8, 8), (32, 32, 32), (16, 16, 16), (-8, -8, -8)),
                                                 ((16, 12, 10), (64, 32, 32), (64, 32, 32), (0, 0, 0))])
 def test_half_3d_fwd_and_xcorr(tin, n, nout, origin):
