@@ -171,16 +171,24 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
                     s_tpl[n] = ENDS ? make_float2(load_cpx(ac, n * a_es).x, load_cpx(a1, n * a_es).x) : load_cpx(ac, n * a_es);
             }
             __syncthreads( );
-            // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q}
+            // input a[n] W_N^{n q}, n = t + 64 i, with W_N^{n q} = W_N^{t q} W_N^{64 i q} (one for
+            // sub-transform 0, whose wave skips the products: the branch is wave-uniform)
             float2 v[E];
 #pragma unroll
             for ( int i = 0; i < E; i++ ) {
                 const int    n = t + T * i;
                 const float2 x = n < item.L ? s_tpl[n] : make_float2(0.f, 0.f);
                 if constexpr ( ! ENDS )
-                    v[i] = cmul(x, win[i]);
+                    v[i] = x;
                 else
                     v[i] = cmul(cmul(x, twm[N + ((T * i * q) & (N - 1))]), lw);
+            }
+            if constexpr ( ! ENDS ) {
+                if ( q != 0 ) {
+#pragma unroll
+                    for ( int i = 0; i < E; i++ )
+                        v[i] = cmul(v[i], win[i]);
+                }
             }
             fft.template run<-1>(v, wl, t);
             if constexpr ( ! ENDS ) {
@@ -222,11 +230,18 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
             }
             fft.template run<+1>(v, wl, t);
             // share e^{2 pi i (o + j) q / N} U_q[(o + j) mod P] of output j = t + 64 i
-            // (sigma already applied on the common path)
+            // (sigma already applied on the common path; no twiddle for sub-transform 0)
+            if constexpr ( ! ENDS ) {
+                if ( q != 0 ) {
+#pragma unroll
+                    for ( int i = 0; i < E; i++ )
+                        v[(i + RS) % E] = cmulc(v[(i + RS) % E], win[i]);
+                }
+            }
 #pragma unroll
             for ( int i = 0; i < E; i++ ) {
                 if constexpr ( ! ENDS ) {
-                    wl[FFT::Layout::slot(t + T * i)] = cmulc(v[(i + RS) % E], win[i]);
+                    wl[FFT::Layout::slot(t + T * i)] = v[(i + RS) % E];
                 }
                 else {
                     float2 c = twm[N + ((T * i * q) & (N - 1))];
