@@ -91,8 +91,23 @@ template <int P, typename TIn>
 static hipError_t go_r2c_packed_cm(const Stage& s, const RowBatchItem* items, int n, int max_rows, float pre,
                                    const float2* tw, hipStream_t st) {
     const dim3 grid((unsigned)((max_rows + kCmRows - 1) / kCmRows), (unsigned)n);
-    k_r2c_rows_packed_cm<P, TIn><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.om.es, s.nout, pre, tw, items);
-    return hipGetLastError( );
+    // non-zero registers of a row's packed input: ceil(L / 128) for the longest template row
+    constexpr int E  = P / 64;
+    const int     nz = (s.L + 127) / 128;
+    auto          go = [&](auto zc) {
+        k_r2c_rows_packed_cm<P, TIn, decltype(zc)::value><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.om.es, s.nout, pre, tw,
+                                                                                               items);
+        return hipGetLastError( );
+    };
+    if ( nz <= 1 )
+        return go(std::integral_constant<int, 1>{ });
+    if ( nz <= 2 )
+        return go(std::integral_constant<int, 2>{ });
+    if ( nz <= 3 && E > 3 )
+        return go(std::integral_constant<int, (E > 3 ? 3 : E)>{ });
+    if ( nz <= 4 && E > 4 )
+        return go(std::integral_constant<int, (E > 4 ? 4 : E)>{ });
+    return go(std::integral_constant<int, E>{ });
 }
 
 hipError_t launch_r2c_rows_batch(const Stage& s, const void* items, int n, int max_rows, RealT in_rt, float pre,

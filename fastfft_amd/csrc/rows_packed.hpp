@@ -204,7 +204,7 @@ constexpr int kCmWaves = 8; // waves per workgroup of the column-major row kerne
 // k_r2c_rows_packed for batches with column-major output (ldc: elements between
 // columns, even).  Grid: ceil(max rows / kCmRows) x templates.  Rows past the
 // template's row count are stored as zeros up to the end of the workgroup's rows.
-template <int P, typename TIn>
+template <int P, typename TIn, int NZ = P / 64>
 __global__ void __launch_bounds__(64 * kCmWaves)
         k_r2c_rows_packed_cm(long ldc, int nout, float pre_scale, const float2* __restrict__ tw,
                              const RowBatchItem* __restrict__ items) {
@@ -252,7 +252,9 @@ __global__ void __launch_bounds__(64 * kCmWaves)
 #pragma unroll
         for ( int i = 0; i < E; i++ )
             v[r][i] = make_float2(v[r][i].x * pre_scale, v[r][i].y * pre_scale);
-        fft.template run<-1>(v[r], wl, t); // v[r][i] = Z[t + 64 i]
+        // registers NZ .. E - 1 hold zeros (no template row is longer than 128 NZ values): the
+        // first pass drops their terms
+        fft.template run<-1, NZ>(v[r], wl, t); // v[r][i] = Z[t + 64 i]
         // Z[P - k] through the region (its last exchange was read before these writes)
 #pragma unroll
         for ( int i = 0; i < E; i++ )

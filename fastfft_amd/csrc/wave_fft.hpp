@@ -276,7 +276,9 @@ struct WaveFFT {
     // One Stockham pass applied to NB independent transforms held by the wave;
     // each transform exchanges through its own LDS region of SLOTS values, so the
     // NB exchanges of a pass share one LDS round trip.
-    template <int Ns, int PI, int DIR, int NB>
+    // NZ: registers NZ .. E - 1 are known to hold zeros on entry (the first pass then
+    // drops their terms at compile time).
+    template <int Ns, int PI, int DIR, int NB, int NZ = E>
     __device__ __forceinline__ void pass(float2 (&v)[NB][E], float2* __restrict__ lds, int t) const {
         if constexpr ( Ns < P ) {
             constexpr int  R    = (P / Ns) < E ? (P / Ns) : E;
@@ -289,7 +291,7 @@ struct WaveFFT {
                     float2 u[R];
 #pragma unroll
                     for ( int r = 0; r < R; r++ )
-                        u[r] = v[nb][m + B * r];
+                        u[r] = (Ns == 1 && m + B * r >= NZ) ? make_float2(0.f, 0.f) : v[nb][m + B * r];
                     if constexpr ( Ns > 1 ) {
 #pragma unroll
                         for ( int r = 1; r < R; r++ ) {
@@ -334,10 +336,11 @@ struct WaveFFT {
         pass<1, 0, DIR, NB>(v, lds, t);
     }
 
-    // In-place transform of one wave-held P-point sequence.  lds: SLOTS values.
-    template <int DIR>
+    // In-place transform of one wave-held P-point sequence.  lds: SLOTS values.  NZ < E:
+    // registers NZ .. E - 1 hold zeros (an input of at most 64 NZ non-zero values).
+    template <int DIR, int NZ = E>
     __device__ __forceinline__ void run(float2 (&v)[E], float2* __restrict__ lds, int t) const {
-        pass<1, 0, DIR, 1>(reinterpret_cast<float2(&)[1][E]>(v), lds, t);
+        pass<1, 0, DIR, 1, NZ>(reinterpret_cast<float2(&)[1][E]>(v), lds, t);
     }
 };
 
