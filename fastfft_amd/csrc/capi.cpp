@@ -627,7 +627,7 @@ static fastfft_status setup_batch_kernels(fastfft_plan p, const std::vector<Stag
     else {
         // candidate shapes (the matrix-core pass has one template group per 32 templates)
         const bool       mixed = batch_mixed(p);
-        std::vector<int> chunks, groups = {1, 2, 4}, ps = {0};
+        std::vector<int> chunks, groups = {1, 2, 4, 8, 16}, ps = {0};
         if ( mixed )
             groups = {0};
         const size_t     per = sizeof(float2) * (slot_elems_a(st[0]) + slot_elems_b(st[2]));
