@@ -337,9 +337,19 @@ template <int P, typename TOut>
 static hipError_t go_c2r_packed_cm(const Stage& s, const RowBatchItem* items, int n, float post, const float2* tw,
                                    hipStream_t st) {
     const dim3 grid((unsigned)((s.nlines + kCmRows - 1) / kCmRows), (unsigned)n);
-    k_c2r_rows_packed_cm<P, TOut><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.im.es, to_device(s.om), s.M, s.o, s.nlines,
-                                                                          post, tw, items);
-    return hipGetLastError( );
+    // a window at origin 0 of at most 128 KEEP outputs needs the first KEEP registers only
+    constexpr int E  = P / 64;
+    const bool    o0 = ((s.o % s.N) + s.N) % s.N == 0;
+    auto          go = [&](auto kc) {
+        k_c2r_rows_packed_cm<P, TOut, decltype(kc)::value><<<grid, dim3(64 * kCmWaves), 0, st>>>(
+                s.im.es, to_device(s.om), s.M, s.o, s.nlines, post, tw, items);
+        return hipGetLastError( );
+    };
+    if ( o0 && s.M <= 256 && E > 2 )
+        return go(std::integral_constant<int, (E > 2 ? 2 : E)>{ });
+    if ( o0 && s.M <= 512 && E > 4 )
+        return go(std::integral_constant<int, (E > 4 ? 4 : E)>{ });
+    return go(std::integral_constant<int, E>{ });
 }
 
 hipError_t launch_c2r_rows_batch(const Stage& s, const void* items, int n, RealT out_rt, float post, const float2* tw,

@@ -289,8 +289,10 @@ __global__ void __launch_bounds__(64 * kCmWaves)
 
 // k_c2r_rows_packed for batches with column-major input (ldc: elements between
 // columns, even; the input column holds at least ceil(nrows / kCmRows) kCmRows
-// values).  Grid: ceil(nrows / kCmRows) x templates.
-template <int P, typename TOut>
+// values).  Grid: ceil(nrows / kCmRows) x templates.  KEEP < E: the window lies in
+// outputs 0 .. 128 KEEP - 1 (origin 0), so registers KEEP .. E - 1 of the inverse are not
+// needed and the last pass does not form them.
+template <int P, typename TOut, int KEEP = P / 64>
 __global__ void __launch_bounds__(64 * kCmWaves)
         k_c2r_rows_packed_cm(long ldc, LineMap om, int M, int o, int nrows, float post_scale, const float2* __restrict__ tw,
                              const RowBatchItem* __restrict__ items) {
@@ -373,7 +375,7 @@ __global__ void __launch_bounds__(64 * kCmWaves)
         auto*              op   = as_global(static_cast<TOut*>(it.out)) + om.base(row);
         unsigned long long best = 0;
 #pragma unroll
-        for ( int i = 0; i < E; i++ ) {
+        for ( int i = 0; i < KEEP; i++ ) {
             const int   n    = 2 * (t + T * i);
             const float y[2] = {vr[i].x * post_scale, vr[i].y * post_scale};
 #pragma unroll
