@@ -2,6 +2,8 @@
 // host memory helpers.  Host code only; kernels live in launch_*.hip.
 #include <hip/hip_runtime.h>
 
+#include <cxxabi.h>
+
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -18,6 +20,67 @@
 #include "tuning.hpp"
 
 using namespace fastfft_amd;
+
+namespace fastfft_amd {
+
+bool log_enabled( ) {
+    static const bool on = [] {
+        const char* v = std::getenv("FASTFFT_LOG");
+        return v && v[0] && std::atoi(v) != 0;
+    }( );
+    return on;
+}
+
+void log_text(const std::string& text) {
+    static std::mutex m;
+    std::lock_guard<std::mutex> lock(m);
+    std::fprintf(stderr, "fastfft: %s%s", text.c_str( ), (! text.empty( ) && text.back( ) == '\n') ? "" : "\n");
+    std::fflush(stderr);
+}
+
+// The demangled tag reads "fastfft_amd::KernelTag<&(void fastfft_amd::k<args>(params))>"
+// for a kernel template and "fastfft_amd::KernelTag<&fastfft_amd::k>" otherwise; the
+// kernel's name with its template arguments is what follows the '&', without the
+// return type and the parameter list.
+void log_launch(const char* tag, dim3 grid, dim3 block, size_t lds) {
+    int         status = 0;
+    char*       d      = abi::__cxa_demangle(tag, nullptr, nullptr, &status);
+    std::string name   = d ? d : tag;
+    std::free(d);
+    const size_t amp = name.find('&');
+    if ( amp != std::string::npos && amp + 1 < name.size( ) && name.back( ) == '>' ) {
+        if ( name[amp + 1] == '(' ) {
+            size_t b = amp + 2;
+            if ( name.compare(b, 5, "void ") == 0 )
+                b += 5;
+            int    depth = 0;
+            size_t e     = b;
+            while ( e < name.size( ) && ! (name[e] == '(' && depth == 0) ) {
+                depth += name[e] == '<' ? 1 : name[e] == '>' ? -1 : 0;
+                e++;
+            }
+            name = name.substr(b, e - b);
+        }
+        else {
+            name = name.substr(amp + 1, name.size( ) - amp - 2);
+        }
+    }
+    for ( size_t at; (at = name.find("fastfft_amd::")) != std::string::npos; )
+        name.erase(at, 13);
+    static const char* const kVectorNames[][2] = {{"HIP_vector_type<float, 2u>", "float2"},
+                                                  {"HIP_vector_type<__half, 2u>", "__half2"}};
+    for ( const auto& r : kVectorNames )
+        for ( size_t at; (at = name.find(r[0])) != std::string::npos; )
+            name.replace(at, std::strlen(r[0]), r[1]);
+    for ( size_t at; (at = name.find(" >")) != std::string::npos; ) // "float2 >" of a nested "<...> >"
+        name.erase(at, 1);
+    char cfg[128];
+    std::snprintf(cfg, sizeof(cfg), " grid=(%u,%u,%u) block=%u lds=%zu", grid.x, grid.y, grid.z, block.x * block.y * block.z,
+                  lds);
+    log_text("launch " + name + cfg);
+}
+
+} // namespace fastfft_amd
 
 namespace {
 
@@ -315,6 +378,8 @@ static fastfft_status get_stages(fastfft_plan p, int opi, const std::vector<Stag
         for ( const Stage& s : st )
             if ( stage_lds_bytes(s.P) > kMaxLdsBytes )
                 return fail(FASTFFT_LDS_EXCEEDED, "stage " + s.ref_kernel + " needs more LDS than a gfx950 CU has");
+        if ( log_enabled( ) )
+            log_text(std::string("plan ") + kOpNames[opi] + " stages:\n" + describe(p->st, st));
         p->stages[opi]       = std::move(st);
         p->stages_built[opi] = true;
     }
@@ -1255,6 +1320,8 @@ fastfft_status fastfft_fwd_image_inv_argmax_noalloc(fastfft_plan plan, void* in,
     return ok( );
 }
 
+static std::string batch_description(fastfft_plan plan);
+
 fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* templates, const int* sizes_xy,
                                  void* const* outputs, int* peaks_xy, float* values) {
     if ( auto rc = check_plan(plan) )
@@ -1316,6 +1383,9 @@ fastfft_status fastfft_set_batch(fastfft_plan plan, int n, const void* const* te
         plan->batch.clear( );
         plan->batch_stages.clear( );
         free_subframe(plan);
+    }
+    else if ( log_enabled( ) ) {
+        log_text("plan " + batch_description(plan));
     }
     return rc;
 }
@@ -1614,6 +1684,30 @@ fastfft_status fastfft_reset_batch_stats(fastfft_plan plan) {
     return e == hipSuccess ? ok( ) : hip_fail(e, "reset_batch_stats");
 }
 
+// The batch set by fastfft_set_batch: its form, chunk size and internal stage lists.
+static std::string batch_description(fastfft_plan plan) {
+    std::string s;
+    s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk) +
+        " group=" + std::to_string(plan->batch_group) + " p=" + std::to_string(plan->batch_p) +
+        " tuned=" + (plan->batch_tuned.empty( ) ? std::string("none") : plan->batch_tuned) +
+        " precision=" + (batch_mixed(plan) ? std::string("split_bf16")
+                         : plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? std::string("fp32 (split_bf16 does not apply)")
+                                                                           : std::string("fp32"));
+    if ( plan->sub_active ) {
+        const bool cm = plan->sub_xc.size( ) == 3 && plan->sub_xc[1].im.es == 1 && plan->sub_xc[1].im.s1 > 1;
+        s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) +
+             (cm ? " slots=column-major" : " slots=row-major") + "\n";
+        s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +
+             "templates:\n" + describe(plan->st, plan->sub_xc);
+    }
+    else {
+        s += " frame=full\n";
+    }
+    if ( plan->stats_on )
+        s.insert(s.find('\n'), " stats=on first_id=" + std::to_string(plan->stats_first_id));
+    return s;
+}
+
 fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, size_t buf_len) {
     if ( auto rc = check_plan(plan) )
         return rc;
@@ -1623,25 +1717,7 @@ fastfft_status fastfft_describe(fastfft_plan plan, const char* op, char* buf, si
         return rc;
     std::string s;
     if ( std::strcmp(op, "batch") == 0 ) {
-        // the batch set by fastfft_set_batch: its form, chunk size and internal stage lists
-        s = "batch templates=" + std::to_string(plan->batch.size( )) + " chunk=" + std::to_string(plan->batch_chunk) +
-            " group=" + std::to_string(plan->batch_group) + " p=" + std::to_string(plan->batch_p) +
-            " tuned=" + (plan->batch_tuned.empty( ) ? std::string("none") : plan->batch_tuned) +
-            " precision=" + (batch_mixed(plan) ? std::string("split_bf16")
-                             : plan->precision == FASTFFT_PRECISION_SPLIT_BF16 ? std::string("fp32 (split_bf16 does not apply)")
-                                                                               : std::string("fp32"));
-        if ( plan->sub_active ) {
-            const bool cm = plan->sub_xc.size( ) == 3 && plan->sub_xc[1].im.es == 1 && plan->sub_xc[1].im.s1 > 1;
-            s += " frame=" + std::to_string(plan->sub_fx) + "x" + std::to_string(plan->sub_fy) +
-                 (cm ? " slots=column-major" : " slots=row-major") + "\n";
-            s += "frame:\n" + describe(plan->st, plan->sub_inv) + "frame spectrum:\n" + describe(plan->st, plan->sub_fwd) +
-                 "templates:\n" + describe(plan->st, plan->sub_xc);
-        }
-        else {
-            s += " frame=full\n";
-        }
-        if ( plan->stats_on )
-            s.insert(s.find('\n'), " stats=on first_id=" + std::to_string(plan->stats_first_id));
+        s = batch_description(plan);
     }
     else {
         const int opi = op_index(op);

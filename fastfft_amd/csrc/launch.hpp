@@ -136,5 +136,13 @@ hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, flo
 // LDS bytes per workgroup of a line stage with sub-transform size P.
 size_t stage_lds_bytes(int P);
 
+// FASTFFT_LOG=1 (read once per process): stderr lines naming each plan's stage
+// lists when they are built, the batch form when it is set, and every kernel
+// launch from the host with its template arguments, grid, block and LDS bytes.
+bool log_enabled( );
+void log_text(const std::string& text);
+// tag: typeid(KernelTag<kernel>).name() (launch_common.hpp)
+void log_launch(const char* tag, dim3 grid, dim3 block, size_t lds);
+
 
 } // namespace fastfft_amd

@@ -3,6 +3,8 @@
 
 #include <cstdlib>
 #include <type_traits>
+#include <typeinfo>
+#include <utility>
 
 #include "launch.hpp"
 #include "line_kernels.hpp"
@@ -57,6 +59,18 @@ hipError_t prepare_launch(K kernel, int nlines, dim3& grid, dim3& block, size_t&
 }
 
 inline int twiddle_stride(int N) { return kTwiddleN / N; }
+
+// A type whose mangled name spells out the kernel K with its template arguments.
+template <auto K>
+struct KernelTag { };
+
+// K<<<grid, block, lds, st>>>(a...), named on stderr first when FASTFFT_LOG is set.
+template <auto K, class... A>
+inline void launch_kernel(dim3 grid, dim3 block, size_t lds, hipStream_t st, A&&... a) {
+    if ( log_enabled( ) )
+        log_launch(typeid(KernelTag<K>).name( ), grid, block, lds);
+    K<<<grid, block, lds, st>>>(std::forward<A>(a)...);
+}
 
 // Integer value of a tuning environment variable (dflt when unset).  Read on
 // every call so that tests can switch kernels within one process; getenv is

@@ -15,7 +15,7 @@ static hipError_t go_fwd(const Stage& s, const StageIO& io, const float2* tw, hi
     hipError_t e = prepare_launch<P>(k_fwd<P, TIn, REAL, TOut>, s.nlines, grid, block, lds);
     if ( e != hipSuccess )
         return e;
-    k_fwd<P, TIn, REAL, TOut><<<grid, block, lds, st>>>(static_cast<const TIn*>(io.src), to_device(s.im), s.L,
+    launch_kernel<k_fwd<P, TIn, REAL, TOut>>(grid, block, lds, st, static_cast<const TIn*>(io.src), to_device(s.im), s.L,
                                                         static_cast<TOut*>(io.dst), to_device(s.om), s.N, s.Q, s.nout,
                                                         s.nlines, io.scale, tw, twiddle_stride(s.N));
     return hipGetLastError( );
@@ -24,7 +24,7 @@ static hipError_t go_fwd(const Stage& s, const StageIO& io, const float2* tw, hi
 // R2C rows with P in {256, 512, 1024}: one block per row, one wave per sub-transform.
 template <int P, int Q, typename TIn>
 static hipError_t go_r2c_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    k_r2c_rows_wave<P, Q, TIn><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_r2c_rows_wave<P, Q, TIn>>(dim3((unsigned)s.nlines), dim3(64 * Q), 0, st,
             static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<float2*>(io.dst), to_device(s.om), s.nout,
             s.nlines, io.scale, tw, nullptr);
     return hipGetLastError( );
@@ -33,7 +33,7 @@ static hipError_t go_r2c_wave(const Stage& s, const StageIO& io, const float2* t
 template <int P, int Q, typename TIn>
 static hipError_t go_r2c_wave_batch(const Stage& s, const RowBatchItem* items, int n, int max_rows, float pre,
                                     const float2* tw, hipStream_t st) {
-    k_r2c_rows_wave<P, Q, TIn><<<dim3((unsigned)max_rows, (unsigned)n), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_r2c_rows_wave<P, Q, TIn>>(dim3((unsigned)max_rows, (unsigned)n), dim3(64 * Q), 0, st,
             nullptr, to_device(s.im), s.L, nullptr, to_device(s.om), s.nout, max_rows, pre, tw, items);
     return hipGetLastError( );
 }
@@ -49,7 +49,7 @@ template <int P, typename TIn>
 static hipError_t go_r2c_packed(const Stage& s, const void* src, void* dst, int nrows, float pre, const float2* tw,
                                 const RowBatchItem* items, int n, hipStream_t st) {
     const dim3 grid((unsigned)((nrows + kPackedRows - 1) / kPackedRows), (unsigned)n);
-    k_r2c_rows_packed<P, TIn><<<grid, dim3(64 * kPackedRows), 0, st>>>(static_cast<const TIn*>(src), to_device(s.im), s.L,
+    launch_kernel<k_r2c_rows_packed<P, TIn>>(grid, dim3(64 * kPackedRows), 0, st, static_cast<const TIn*>(src), to_device(s.im), s.L,
                                                                       static_cast<float2*>(dst), to_device(s.om), s.nout,
                                                                       nrows, pre, tw, items);
     return hipGetLastError( );
@@ -95,7 +95,7 @@ static hipError_t go_r2c_packed_cm(const Stage& s, const RowBatchItem* items, in
     constexpr int E  = P / 64;
     const int     nz = (s.L + 127) / 128;
     auto          go = [&](auto zc) {
-        k_r2c_rows_packed_cm<P, TIn, decltype(zc)::value><<<grid, dim3(64 * kCmWaves), 0, st>>>(s.om.es, s.nout, pre, tw,
+        launch_kernel<k_r2c_rows_packed_cm<P, TIn, decltype(zc)::value>>(grid, dim3(64 * kCmWaves), 0, st, s.om.es, s.nout, pre, tw,
                                                                                                items);
         return hipGetLastError( );
     };
@@ -187,21 +187,21 @@ static hipError_t go_full_wave(const Stage& s, const StageIO& io, const float2* 
     const dim3 grid((unsigned)s.nlines), block(64 * Q);
     if ( io.src_real ) {
         if ( io.src_rt == RealT::f32 )
-            k_fwd_full_wave<P, Q, float, true><<<grid, block, 0, st>>>(static_cast<const float*>(io.src), to_device(s.im),
+            launch_kernel<k_fwd_full_wave<P, Q, float, true>>(grid, block, 0, st, static_cast<const float*>(io.src), to_device(s.im),
                                                                        static_cast<float2*>(io.dst), to_device(s.om),
                                                                        s.nout, s.nlines, io.scale, tw);
         else
-            k_fwd_full_wave<P, Q, __half, true><<<grid, block, 0, st>>>(static_cast<const __half*>(io.src), to_device(s.im),
+            launch_kernel<k_fwd_full_wave<P, Q, __half, true>>(grid, block, 0, st, static_cast<const __half*>(io.src), to_device(s.im),
                                                                         static_cast<float2*>(io.dst), to_device(s.om),
                                                                         s.nout, s.nlines, io.scale, tw);
     }
     else {
         if ( io.src_ct == CpxT::c64 )
-            k_fwd_full_wave<P, Q, float2, false><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+            launch_kernel<k_fwd_full_wave<P, Q, float2, false>>(grid, block, 0, st, static_cast<const float2*>(io.src), to_device(s.im),
                                                                          static_cast<float2*>(io.dst), to_device(s.om),
                                                                          s.nout, s.nlines, io.scale, tw);
         else
-            k_fwd_full_wave<P, Q, __half2, false><<<grid, block, 0, st>>>(static_cast<const __half2*>(io.src),
+            launch_kernel<k_fwd_full_wave<P, Q, __half2, false>>(grid, block, 0, st, static_cast<const __half2*>(io.src),
                                                                           to_device(s.im), static_cast<float2*>(io.dst),
                                                                           to_device(s.om), s.nout, s.nlines, io.scale, tw);
     }
@@ -214,11 +214,11 @@ template <int P, int Q>
 static hipError_t go_full_wave_pair(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const dim3 grid((unsigned)(s.nlines / 2)), block(64 * Q);
     if ( io.src_rt == RealT::f32 )
-        k_fwd_full_wave_pair<P, Q, float><<<grid, block, 0, st>>>(static_cast<const float*>(io.src), to_device(s.im),
+        launch_kernel<k_fwd_full_wave_pair<P, Q, float>>(grid, block, 0, st, static_cast<const float*>(io.src), to_device(s.im),
                                                                   static_cast<float2*>(io.dst), to_device(s.om), s.nout,
                                                                   s.nlines / 2, io.scale, tw);
     else
-        k_fwd_full_wave_pair<P, Q, __half><<<grid, block, 0, st>>>(static_cast<const __half*>(io.src), to_device(s.im),
+        launch_kernel<k_fwd_full_wave_pair<P, Q, __half>>(grid, block, 0, st, static_cast<const __half*>(io.src), to_device(s.im),
                                                                    static_cast<float2*>(io.dst), to_device(s.om), s.nout,
                                                                    s.nlines / 2, io.scale, tw);
     return hipGetLastError( );

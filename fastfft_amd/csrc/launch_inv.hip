@@ -14,7 +14,7 @@ static hipError_t go_inv(const Stage& s, const StageIO& io, const float2* tw, hi
     hipError_t e = prepare_launch<P>(k_inv<P, TIn, HERM, TOut, REAL_OUT>, s.nlines, grid, block, lds);
     if ( e != hipSuccess )
         return e;
-    k_inv<P, TIn, HERM, TOut, REAL_OUT><<<grid, block, lds, st>>>(static_cast<const TIn*>(io.src), to_device(s.im),
+    launch_kernel<k_inv<P, TIn, HERM, TOut, REAL_OUT>>(grid, block, lds, st, static_cast<const TIn*>(io.src), to_device(s.im),
                                                                   static_cast<TOut*>(io.dst), to_device(s.om), s.N, s.M,
                                                                   s.o, s.Q, s.nlines, io.scale, tw, twiddle_stride(s.N));
     return hipGetLastError( );
@@ -31,7 +31,7 @@ template <int P, typename TOut>
 static hipError_t go_c2r_packed(const Stage& s, const void* src, void* dst, float post, const float2* tw,
                                 unsigned long long* keys, const RowBatchItem* items, int n, hipStream_t st) {
     const dim3 grid((unsigned)((s.nlines + kPackedRows - 1) / kPackedRows), (unsigned)n);
-    k_c2r_rows_packed<P, TOut><<<grid, dim3(64 * kPackedRows), 0, st>>>(static_cast<const float2*>(src), to_device(s.im),
+    launch_kernel<k_c2r_rows_packed<P, TOut>>(grid, dim3(64 * kPackedRows), 0, st, static_cast<const float2*>(src), to_device(s.im),
                                                                        static_cast<TOut*>(dst), to_device(s.om), s.M, s.o,
                                                                        s.nlines, post, tw, keys, items);
     return hipGetLastError( );
@@ -104,11 +104,11 @@ hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, floa
     const bool  cm = c2r_cm_input(s);
     auto        go = [&](auto pc, auto hc) {
         if ( cm )
-            k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value, true><<<grid, block, 0, st>>>(
+            launch_kernel<k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value, true>>(grid, block, 0, st,
                     to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq,
                     bs.part);
         else
-            k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value><<<grid, block, 0, st>>>(
+            launch_kernel<k_c2r_stats_packed<decltype(pc)::value, decltype(hc)::value>>(grid, block, 0, st,
                     to_device(s.im), s.M, s.o, s.nlines, post, tw, it, n, first_id, bs.mip, bs.best, bs.sum, bs.sum_sq,
                     bs.part);
         return hipGetLastError( );
@@ -121,7 +121,7 @@ hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, floa
     if ( e != hipSuccess || G == 1 )
         return e;
     const long npix = (long)s.nlines * s.M;
-    k_stats_merge<<<dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st>>>(bs.part, G, npix, bs.mip, bs.best, bs.sum,
+    launch_kernel<k_stats_merge>(dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, bs.part, G, npix, bs.mip, bs.best, bs.sum,
                                                                              bs.sum_sq);
     return hipGetLastError( );
 }
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) k_stats_reset(float* mip, int* best, floa
 hipError_t launch_stats_reset(const BatchStats& bs, long n, hipStream_t st) {
     if ( n < 1 )
         return hipSuccess;
-    k_stats_reset<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(bs.mip, bs.best, bs.sum, bs.sum_sq, n);
+    launch_kernel<k_stats_reset>(dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bs.mip, bs.best, bs.sum, bs.sum_sq, n);
     return hipGetLastError( );
 }
 
@@ -185,11 +185,11 @@ static hipError_t try_c2r_window_packed(const Stage& s, const StageIO& io, const
         constexpr int Q = decltype(qc)::value;
         const dim3    grid((unsigned)s.nlines), block(64 * Q);
         if ( io.dst_rt == RealT::f32 )
-            k_c2r_window_packed<512, Q, float><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+            launch_kernel<k_c2r_window_packed<512, Q, float>>(grid, block, 0, st, static_cast<const float2*>(io.src), to_device(s.im),
                                                                        static_cast<float*>(io.dst), to_device(s.om), s.M,
                                                                        s.o, s.nlines, io.scale, tw, rk);
         else
-            k_c2r_window_packed<512, Q, __half><<<grid, block, 0, st>>>(static_cast<const float2*>(io.src), to_device(s.im),
+            launch_kernel<k_c2r_window_packed<512, Q, __half>>(grid, block, 0, st, static_cast<const float2*>(io.src), to_device(s.im),
                                                                         static_cast<__half*>(io.dst), to_device(s.om), s.M,
                                                                         s.o, s.nlines, io.scale, tw, rk);
         return hipGetLastError( );
@@ -238,13 +238,13 @@ hipError_t launch_c2r_r2c_rows(const Stage& c, const Stage& f, const float2* src
     auto go = [&](auto qc) -> hipError_t {
         constexpr int Q = decltype(qc)::value;
         if ( rb == 1 )
-            k_c2r_r2c_rows<Q><<<dim3((unsigned)c.nlines), dim3(64 * Q), 0, st>>>(
+            launch_kernel<k_c2r_r2c_rows<Q>>(dim3((unsigned)c.nlines), dim3(64 * Q), 0, st,
                     src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
         else if ( rb == 2 )
-            k_c2r_r2c_rows_tile<Q, 2><<<dim3((unsigned)(c.nlines / 2)), dim3(64 * Q * 2), 0, st>>>(
+            launch_kernel<k_c2r_r2c_rows_tile<Q, 2>>(dim3((unsigned)(c.nlines / 2)), dim3(64 * Q * 2), 0, st,
                     src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
         else if constexpr ( Q <= 4 )
-            k_c2r_r2c_rows_tile<Q, 4><<<dim3((unsigned)(c.nlines / 4)), dim3(64 * Q * 4), 0, st>>>(
+            launch_kernel<k_c2r_r2c_rows_tile<Q, 4>>(dim3((unsigned)(c.nlines / 4)), dim3(64 * Q * 4), 0, st,
                     src, to_device(c.im), c.M, c.o, c.nlines, post, dst, to_device(f.om), f.nout, pre, tw);
         return hipGetLastError( );
     };
@@ -259,7 +259,7 @@ template <int P, int Q, int RS, typename TOut>
 static hipError_t go_c2r_wave(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     // the argmax keys need one slot per row
     const bool keys = io.amax && s.nlines <= kArgmaxPartials;
-    k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_c2r_rows_wave<P, Q, RS, TOut>>(dim3((unsigned)s.nlines), dim3(64 * Q), 0, st,
             static_cast<const float2*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
             s.nlines, io.scale, tw, keys ? io.amax->keys : nullptr, nullptr);
     hipError_t e = hipGetLastError( );
@@ -311,7 +311,7 @@ static hipError_t try_c2r_wave(const Stage& s, const StageIO& io, const float2* 
 template <int P, int Q, int RS, typename TOut>
 static hipError_t go_c2r_wave_batch(const Stage& s, const RowBatchItem* items, int n, float post, const float2* tw,
                                     hipStream_t st) {
-    k_c2r_rows_wave<P, Q, RS, TOut><<<dim3((unsigned)s.nlines, (unsigned)n), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_c2r_rows_wave<P, Q, RS, TOut>>(dim3((unsigned)s.nlines, (unsigned)n), dim3(64 * Q), 0, st,
             nullptr, to_device(s.im), nullptr, to_device(s.om), s.M, s.o, s.nlines, post, tw, nullptr, items);
     return hipGetLastError( );
 }
@@ -341,7 +341,7 @@ static hipError_t go_c2r_packed_cm(const Stage& s, const RowBatchItem* items, in
     constexpr int E  = P / 64;
     const bool    o0 = ((s.o % s.N) + s.N) % s.N == 0;
     auto          go = [&](auto kc) {
-        k_c2r_rows_packed_cm<P, TOut, decltype(kc)::value><<<grid, dim3(64 * kCmWaves), 0, st>>>(
+        launch_kernel<k_c2r_rows_packed_cm<P, TOut, decltype(kc)::value>>(grid, dim3(64 * kCmWaves), 0, st,
                 s.im.es, to_device(s.om), s.M, s.o, s.nlines, post, tw, items);
         return hipGetLastError( );
     };
@@ -411,18 +411,16 @@ static hipError_t go_full_inv(const Stage& s, const StageIO& io, const float2* t
     if ( s.kind == StageKind::inv_c2r ) {
         const auto* in = static_cast<const float2*>(io.src);
         if ( io.dst_rt == RealT::f32 )
-            k_inv_full_wave<P, Q, float2, true, float, true>
-                    <<<grid, block, 0, st>>>(in, im, static_cast<float*>(io.dst), om, s.nlines, io.scale, tw);
+            launch_kernel<k_inv_full_wave<P, Q, float2, true, float, true>>(grid, block, 0, st, in, im, static_cast<float*>(io.dst), om, s.nlines, io.scale, tw);
         else
-            k_inv_full_wave<P, Q, float2, true, __half, true>
-                    <<<grid, block, 0, st>>>(in, im, static_cast<__half*>(io.dst), om, s.nlines, io.scale, tw);
+            launch_kernel<k_inv_full_wave<P, Q, float2, true, __half, true>>(grid, block, 0, st, in, im, static_cast<__half*>(io.dst), om, s.nlines, io.scale, tw);
     }
     else if ( io.src_ct == CpxT::c64 ) {
-        k_inv_full_wave<P, Q, float2, false, float2, false><<<grid, block, 0, st>>>(
+        launch_kernel<k_inv_full_wave<P, Q, float2, false, float2, false>>(grid, block, 0, st,
                 static_cast<const float2*>(io.src), im, static_cast<float2*>(io.dst), om, s.nlines, io.scale, tw);
     }
     else {
-        k_inv_full_wave<P, Q, __half2, false, float2, false><<<grid, block, 0, st>>>(
+        launch_kernel<k_inv_full_wave<P, Q, __half2, false, float2, false>>(grid, block, 0, st,
                 static_cast<const __half2*>(io.src), im, static_cast<float2*>(io.dst), om, s.nlines, io.scale, tw);
     }
     return hipGetLastError( );
@@ -448,7 +446,7 @@ static hipError_t try_full_inv(const Stage& s, const StageIO& io, const float2* 
 // or 256 for N = 256, whatever the stage's own sub-transform size).
 template <int P, int Q, typename TIn, bool HERM, typename TOut, bool REAL_OUT>
 static hipError_t go_window(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    k_inv_window_rows<P, Q, TIn, HERM, TOut, REAL_OUT><<<dim3((unsigned)s.nlines), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_inv_window_rows<P, Q, TIn, HERM, TOut, REAL_OUT>>(dim3((unsigned)s.nlines), dim3(64 * Q), 0, st,
             static_cast<const TIn*>(io.src), to_device(s.im), static_cast<TOut*>(io.dst), to_device(s.om), s.M, s.o,
             s.nlines, io.scale, tw);
     return hipGetLastError( );

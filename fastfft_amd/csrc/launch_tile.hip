@@ -7,7 +7,7 @@ namespace fastfft_amd {
 
 template <int N, int DIR, typename TIn, bool HERM, typename TOut, bool TI, bool TO>
 static hipError_t go_tile(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st, int M, int o) {
-    k_tile_lines<N, DIR, TIn, HERM, TOut, TI, TO><<<dim3((unsigned)(s.nlines / tile_workgroup_lines<TI, TO>)), dim3(512), 0, st>>>(
+    launch_kernel<k_tile_lines<N, DIR, TIn, HERM, TOut, TI, TO>>(dim3((unsigned)(s.nlines / tile_workgroup_lines<TI, TO>)), dim3(512), 0, st,
             static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<TOut*>(io.dst), to_device(s.om), M, o,
             s.nlines, io.scale, tw);
     return hipGetLastError( );

@@ -21,7 +21,7 @@ static hipError_t go_xcorr(const Stage& s, const StageIO& io, const float2* tw, 
     hipError_t e = prepare_launch<P>(k_xcorr<P, TImg>, s.nlines, grid, block, lds);
     if ( e != hipSuccess )
         return e;
-    k_xcorr<P, TImg><<<grid, block, lds, st>>>(static_cast<const float2*>(io.src), to_device(s.im), s.L,
+    launch_kernel<k_xcorr<P, TImg>>(grid, block, lds, st, static_cast<const float2*>(io.src), to_device(s.im), s.L,
                                                 static_cast<const TImg*>(io.img), to_device(s.gm),
                                                 static_cast<float2*>(io.dst), to_device(s.om), s.N, s.M, s.o, s.Q,
                                                 s.nlines, io.scale, tw, twiddle_stride(s.N));
@@ -49,7 +49,7 @@ static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* 
         groups        = std::max(1, std::min(groups, io.nxitems));
     }
     XcorrItem one{io.src, io.dst, s.L, 0};
-    k_xcorr_col<P, Q, RS, TImg><<<dim3((unsigned)blocks, (unsigned)groups), dim3(64 * Q), 0, st>>>(
+    launch_kernel<k_xcorr_col<P, Q, RS, TImg>>(dim3((unsigned)blocks, (unsigned)groups), dim3(64 * Q), 0, st,
             one, static_cast<const XcorrItem*>(io.xitems), io.nxitems, s.im.s1, s.im.es, static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M,
             s.o, io.scale, tw, s.nlines - 1, pack);
     return hipGetLastError( );
@@ -115,7 +115,7 @@ static hipError_t go_xcorr_toeplitz(const Stage& s, const StageIO& io, hipStream
     const int n = io.nxitems;
     if ( ! io.xitems || n < 1 || io.xout_step <= 0 )
         return hipErrorNotSupported;
-    k_xcorr_toeplitz<<<dim3((unsigned)(s.nlines - 1)), dim3(kTzWaves * 64), 0, st>>>(
+    launch_kernel<k_xcorr_toeplitz>(dim3((unsigned)(s.nlines - 1)), dim3(kTzWaves * 64), 0, st,
             static_cast<const XcorrItem*>(io.xitems), n, s.im.s1, static_cast<const float2*>(io.img), s.gm.s1, s.om.s1,
             io.xout_step, s.M, s.L, io.scale * (float)s.N, s.nlines - 1);
     return hipGetLastError( );

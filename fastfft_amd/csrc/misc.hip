@@ -40,7 +40,7 @@ hipError_t twiddle_table_address(float2** out) {
 }
 
 hipError_t launch_twiddle_init(float2* tw, hipStream_t stream) {
-    k_twiddle_init<<<dim3((kTwiddleTableN + 255) / 256), dim3(256), 0, stream>>>(tw);
+    launch_kernel<k_twiddle_init>(dim3((kTwiddleTableN + 255) / 256), dim3(256), 0, stream, tw);
     return hipGetLastError( );
 }
 
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(kArgmaxThreads)
 
 hipError_t launch_argmax_keys(const unsigned long long* keys, int n, int nx, float* result_value, int* result_xy,
                               hipStream_t stream, int batch, long key_stride) {
-    k_argmax_keys<<<batch, kArgmaxThreads, 0, stream>>>(keys, n, nx, result_value, result_xy, key_stride);
+    launch_kernel<k_argmax_keys>(batch, kArgmaxThreads, 0, stream, keys, n, nx, result_value, result_xy, key_stride);
     return hipGetLastError( );
 }
 
@@ -163,13 +163,13 @@ hipError_t launch_argmax(const void* img, RealT rt, int nx, int ny, int pitch, f
     float* pv         = partial_v;
     int*   pi         = partial_i;
     if ( rt == RealT::f32 )
-        k_argmax_partial<float><<<blocks, kArgmaxThreads, 0, stream>>>(static_cast<const float*>(img), nx, ny, pitch, pv, pi);
+        launch_kernel<k_argmax_partial<float>>(blocks, kArgmaxThreads, 0, stream, static_cast<const float*>(img), nx, ny, pitch, pv, pi);
     else
-        k_argmax_partial<__half><<<blocks, kArgmaxThreads, 0, stream>>>(static_cast<const __half*>(img), nx, ny, pitch, pv, pi);
+        launch_kernel<k_argmax_partial<__half>>(blocks, kArgmaxThreads, 0, stream, static_cast<const __half*>(img), nx, ny, pitch, pv, pi);
     hipError_t e = hipGetLastError( );
     if ( e != hipSuccess )
         return e;
-    k_argmax_final<<<1, kArgmaxThreads, 0, stream>>>(pv, pi, blocks, nx, result_value, result_xy);
+    launch_kernel<k_argmax_final>(1, kArgmaxThreads, 0, stream, pv, pi, blocks, nx, result_value, result_xy);
     return hipGetLastError( );
 }
 

@@ -518,3 +518,41 @@ def test_fused_argmax_graph_replays():
         ft.FwdImageInvFFTArgMax(a, img, out, peak, val)  # eager call on the same plan between replays
         torch.cuda.synchronize()
         assert np.array_equal(H.to_host(peak), want_peak), rep
+
+
+def test_log_names_kernel_launches():
+    """FASTFFT_LOG=1: every kernel launch of a FwdImageInvFFT is named on stderr with its template
+    arguments and launch shape (run in a child process, since the flag is read once per process)."""
+    import os
+    import re
+    import subprocess
+    import sys
+
+    H.log("FASTFFT_LOG=1 kernel launch lines of a 64 -> 512 -> 128 correlation")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import torch\n"
+            "import fastfft_amd as F\n"
+            "d = torch.device('cuda:0')\n"
+            "ft = F.FourierTransformer()\n"
+            "ft.SetForwardFFTPlan(64, 64, 1, 512, 512, 1)\n"
+            "ft.SetInverseFFTPlan(512, 512, 1, 128, 128, 1)\n"
+            "ft.SetOutputWindow(-64, -64)\n"
+            "a = torch.ones(max(ft.ReturnInputMemorySize(), ft.ReturnInvOutputMemorySize()), device=d)\n"
+            "img = torch.ones(ft.ReturnFwdOutputMemorySize(), device=d)\n"
+            "out = torch.zeros(ft.ReturnInvOutputMemorySize(), device=d)\n"
+            "ft.FwdImageInvFFT(a, img, out)\n"
+            "torch.cuda.synchronize()\n")
+    env = dict(os.environ, FASTFFT_LOG="1")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    launches = [l for l in r.stderr.splitlines() if l.startswith("fastfft: launch ")]
+    print("\n".join(launches))
+    assert "fastfft: plan xcorr stages:" in r.stderr
+    pat = re.compile(r"^fastfft: launch (k_\w+)(<.*>)? grid=\(\d+,\d+,\d+\) block=\d+ lds=\d+$")
+    names = []
+    for l in launches:
+        m = pat.match(l)
+        assert m, l
+        names.append(m.group(1))
+    # the twiddle table once, then one kernel per stage of the three-stage correlation
+    assert names[0] == "k_twiddle_init" and len(names) >= 4

@@ -224,3 +224,38 @@ def test_noalloc_rejects_plans_beyond_static_scratch():
     with pytest.raises(F.FastFFTError) as e:
         f3.FwdImageInvFFTArgMaxNoAlloc(1, 2, 3, 4, 5)
     assert e.value.status == F.UNSUPPORTED
+
+
+def test_log_names_stage_lists():
+    """FASTFFT_LOG=1 writes each operation's stage list to stderr when it is built (no GPU needed
+    for planning); without it the library writes nothing."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import fastfft_amd as F\n"
+            "ft = F.FourierTransformer()\n"
+            "ft.SetForwardFFTPlan(384, 384, 1, 4096, 4096, 1)\n"
+            "ft.SetInverseFFTPlan(4096, 4096, 1, 512, 512, 1)\n"
+            "ft.SetOutputWindow(-256, -256)\n"
+            "ft.Describe('xcorr')\n"
+            "ft.Describe('xcorr')\n"
+            "ft.Describe('fwd')\n")
+
+    def run(flag):
+        env = dict(os.environ, FASTFFT_LOG=flag)
+        r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        return [l for l in r.stderr.splitlines() if l.startswith("fastfft: ") or l.startswith("fwd") or
+                l.startswith("xcorr") or l.startswith("inv")]
+
+    lines = run("1")
+    heads = [l for l in lines if l.startswith("fastfft: plan ")]
+    # built once per operation: the second Describe('xcorr') reuses the list
+    assert heads == ["fastfft: plan xcorr stages:", "fastfft: plan fwd stages:"]
+    i = lines.index("fastfft: plan xcorr stages:")
+    kinds = [l.split()[0] for l in lines[i + 1:i + 4]]
+    assert kinds == ["fwd_real", "xcorr", "inv_c2r"]
+    assert "N=4096 P=512 Q=8 L=384 M=512 o=-256" in lines[i + 2]
+    assert run("0") == []
