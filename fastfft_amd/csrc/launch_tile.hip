@@ -1,15 +1,30 @@
 // Line transforms of 256 or 512 points, 16 lines per workgroup (tile_lines.hpp),
 // for the passes of the 3-D transforms.
+#include <cstdint>
+
 #include "launch_common.hpp"
 #include "tile_lines.hpp"
 
 namespace fastfft_amd {
 
+// 16-byte accesses to two neighbouring lines of a tile: even line offsets within
+// the tile's rows and a 16-byte aligned buffer
+static bool pair_map(const HostLineMap& m, const void* p) {
+    return m.es % 2 == 0 && (m.n2 == 1 || m.s1 % 2 == 0) && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
 template <int N, int DIR, typename TIn, bool HERM, typename TOut, bool TI, bool TO>
 static hipError_t go_tile(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st, int M, int o) {
-    launch_kernel<k_tile_lines<N, DIR, TIn, HERM, TOut, TI, TO>>(dim3((unsigned)(s.nlines / tile_workgroup_lines<TI, TO>)), dim3(512), 0, st,
-            static_cast<const TIn*>(io.src), to_device(s.im), s.L, static_cast<TOut*>(io.dst), to_device(s.om), M, o,
-            s.nlines, io.scale, tw);
+    // bit 0: XCD-contiguous tiles, measured faster where a tile writes more rows than it reads (a
+    // forward that pads) and slower for reading tiles; bit 1: 16-byte tile accesses where aligned.
+    // FASTFFT_TILE_FLAGS overrides both.
+    const int tf    = env_int("FASTFFT_TILE_FLAGS", (DIR < 0 && M > s.L) ? 3 : 2);
+    int       flags = (TI || TO) ? (tf & 1) : 0;
+    if ( (tf & 2) && (! TI || pair_map(s.im, io.src)) && (! TO || pair_map(s.om, io.dst)) )
+        flags |= 2;
+    launch_kernel<k_tile_lines<N, DIR, TIn, HERM, TOut, TI, TO>>(
+            dim3((unsigned)(s.nlines / tile_workgroup_lines<TI, TO>)), dim3(512), 0, st, static_cast<const TIn*>(io.src),
+            to_device(s.im), s.L, static_cast<TOut*>(io.dst), to_device(s.om), M, o, s.nlines, io.scale, tw, flags);
     return hipGetLastError( );
 }
 

@@ -170,13 +170,16 @@ def test_full_inv_512_to_128_sampled_and_cropped():
 
 
 # ------------------------------------------------------------------ tiled z-passes (N = 256, 512)
-@pytest.mark.parametrize("mode", ["default", "no_tile"])
+@pytest.mark.parametrize("mode", ["default", "no_tile", "flags0", "flags3"])
 @pytest.mark.parametrize("vin,n", [((20, 30, 40), (64, 64, 512)), ((32, 32, 32), (64, 64, 256))])
 def test_fwd_3d_tiled_z_pass(vin, n, mode, monkeypatch):
     """Forward transforms whose z-pass (adjacent lines, far-apart elements) runs as tiles of 16 lines,
-    against the FP64 oracle, with and without the tiled kernel."""
+    against the FP64 oracle, with and without the tiled kernel, and with its tile order and 16-byte
+    tile accesses forced off (flags0) and on (flags3)."""
     if mode == "no_tile":
         monkeypatch.setenv("FASTFFT_NO_TILE_LINES", "1")
+    if mode.startswith("flags"):
+        monkeypatch.setenv("FASTFFT_TILE_FLAGS", mode[-1])
     H.log(f"3d fwd {vin} -> {n}, z-pass mode={mode}")
     v = _vol(vin[::-1], seed=sum(vin) + 1)
     got = _fwd_gpu(v, n)
@@ -184,12 +187,15 @@ def test_fwd_3d_tiled_z_pass(vin, n, mode, monkeypatch):
     assert O.rel_l2(got, want) < TOL
 
 
-@pytest.mark.parametrize("mode", ["default", "no_tile", "z_first"])
+@pytest.mark.parametrize("mode", ["default", "no_tile", "z_first", "flags0", "flags3"])
 def test_inv_3d_tiled_z_pass_window(mode, monkeypatch):
     """A trimmed inverse whose z-pass keeps 100 of 512 lags at an origin off the 64-lane grid (after the
-    y-pass by default, before it with FASTFFT_INV3D_Z_FIRST=1)."""
+    y-pass by default, before it with FASTFFT_INV3D_Z_FIRST=1; tile order and 16-byte tile accesses
+    forced off and on with FASTFFT_TILE_FLAGS)."""
     if mode == "no_tile":
         monkeypatch.setenv("FASTFFT_NO_TILE_LINES", "1")
+    if mode.startswith("flags"):
+        monkeypatch.setenv("FASTFFT_TILE_FLAGS", mode[-1])
     if mode == "z_first":
         monkeypatch.setenv("FASTFFT_INV3D_Z_FIRST", "1")
     n, nout, origin = (128, 64, 512), (100, 60, 100), (-50, -30, -37)
