@@ -113,48 +113,56 @@ __global__ void __launch_bounds__(64 * Q)
     }
 }
 
-// Two real lines per workgroup as one complex line z = a + i b (the R2C x-pass of a
-// FwdFFT without size change, the reference's block_fft_kernel_R2C_NONE_XY,
+// Two real lines as one complex line z = a + i b (the R2C x-pass of a FwdFFT
+// without size change, the reference's block_fft_kernel_R2C_NONE_XY,
 // FastFFT.cu:1055-1077, which computes one real line as a complex one):
 //   Z = FFT_N(z),  A[k] = (Z[k] + conj Z[N-k]) / 2,  B[k] = -i (Z[k] - conj Z[N-k]) / 2,
 // so a pair costs one N-point transform.  The output is the transposed spectrum
 // (om.s1 = 1: lines 2 p and 2 p + 1 adjacent), and A[k], B[k] leave as one 16-byte
-// store per k (om.es even, out 16-byte aligned).  Grid: nlines / 2 workgroups.
-template <int P, int Q, typename TIn>
-__global__ void __launch_bounds__(64 * Q)
+// store per k (om.es even, out 16-byte aligned).  A workgroup takes PPW
+// neighbouring pairs, Q waves each, and stores the 2 PPW values of each k as one
+// 16 PPW-byte segment.  Grid: npairs / PPW workgroups.
+template <int P, int Q, typename TIn, int PPW = 1>
+__global__ void __launch_bounds__(64 * Q * PPW)
         k_fwd_full_wave_pair(const TIn* __restrict__ in, LineMap im, float2* __restrict__ out, LineMap om, int nout,
                              int npairs, float pre_scale, const float2* __restrict__ tw) {
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q;
     using FFT       = WaveFFT<P>;
     static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
-    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ float2 lds_all[PPW][Q * FFT::SLOTS];
     __shared__ float2 s_twp[P];
     __shared__ float2 s_twn[NT];
-    const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
-    const int     t    = threadIdx.x % T;
-    const int     pair = xcd_row(blockIdx.x, npairs);
-    const float2* twm  = multi_twiddles(tw);
+    const int     g     = __builtin_amdgcn_readfirstlane(threadIdx.x / NT); // pair of the workgroup
+    const int     tid   = threadIdx.x % NT;
+    const int     q     = __builtin_amdgcn_readfirstlane(tid / T);
+    const int     t     = tid % T;
+    const int     pair0 = xcd_row(blockIdx.x, npairs / PPW) * PPW;
+    const int     pair  = pair0 + g;
+    const float2* twm   = multi_twiddles(tw);
+    float2*       lds   = lds_all[g];
     // ---- the pair into LDS as one complex line (natural order), every load of the
-    // thread in flight before its LDS writes
+    // thread in flight before its LDS writes; the first pair's waves stage the tables
     {
         RowTwiddleStage<P, Q> tws;
-        tws.load(twm);
+        if ( g == 0 )
+            tws.load(twm);
         const auto* la = as_global(in) + im.base(2 * pair);
         const auto* lb = as_global(in) + im.base(2 * pair + 1);
         float2      z[N / NT];
 #pragma unroll
         for ( int r = 0; r < N / NT; r++ ) {
-            const int n = threadIdx.x + NT * r;
+            const int n = tid + NT * r;
             z[r]        = make_float2(load_real(la, n * im.es), load_real(lb, n * im.es));
         }
-        tws.store(s_twp, s_twn);
+        if ( g == 0 )
+            tws.store(s_twp, s_twn);
 #pragma unroll
         for ( int r = 0; r < N / NT; r++ )
-            lds[threadIdx.x + NT * r] = make_float2(z[r].x * pre_scale, z[r].y * pre_scale);
+            lds[tid + NT * r] = make_float2(z[r].x * pre_scale, z[r].y * pre_scale);
     }
     __syncthreads( );
     // ---- radix-Q pass: thread m holds x[s P + m], s < Q, and leaves y_q[m] at q P + m
-    for ( int m = threadIdx.x; m < P; m += NT ) {
+    for ( int m = tid; m < P; m += NT ) {
         float2 u[Q];
 #pragma unroll
         for ( int s = 0; s < Q; s++ )
@@ -174,17 +182,18 @@ __global__ void __launch_bounds__(64 * Q)
         v[i] = cmul(cmul(lds[q * P + t + T * i], twm[N + ((T * i * q) & (N - 1))]), lw);
     __syncthreads( );
     fft.template run<-1>(v, lds + q * FFT::SLOTS, t);
-    // ---- Z in natural order through LDS, then A[k], B[k] of the two lines
+    // ---- Z in natural order through LDS, then A[k], B[k] of the 2 PPW lines
     __syncthreads( );
 #pragma unroll
     for ( int i = 0; i < E; i++ )
         lds[dec_slot<P, Q>(Q * (t + T * i) + q)] = v[i];
     __syncthreads( );
-    auto* op = as_global(out) + om.base(2 * pair);
-    for ( int k = threadIdx.x; k < nout; k += NT ) {
-        const float2 zk = lds[dec_slot<P, Q>(k)];
-        const float2 zn = lds[dec_slot<P, Q>((N - k) & (N - 1))];
-        store_cpx_pair(op + (long)k * om.es, 0, make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y)),
+    auto* op = as_global(out) + om.base(2 * pair0);
+    for ( int j = threadIdx.x; j < nout * PPW; j += NT * PPW ) {
+        const int    k  = j / PPW, h = j % PPW;
+        const float2 zk = lds_all[h][dec_slot<P, Q>(k)];
+        const float2 zn = lds_all[h][dec_slot<P, Q>((N - k) & (N - 1))];
+        store_cpx_pair(op + (long)k * om.es, h, make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y)),
                        make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x)));
     }
 }

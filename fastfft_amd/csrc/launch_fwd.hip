@@ -210,18 +210,28 @@ static hipError_t go_full_wave(const Stage& s, const StageIO& io, const float2* 
 
 // Full-length real lines into a transposed spectrum (lines adjacent, even element
 // step, 16-byte aligned output): two lines per workgroup as one complex line.
+// PPW pairs per workgroup (FASTFFT_PAIR_GROUP: 1 or 2; 2 where the pair count allows
+// and a workgroup of 2 Q waves fits).
+template <int P, int Q, int PPW>
+static hipError_t go_full_wave_pair_g(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
+    const int  npairs = s.nlines / 2;
+    const dim3 grid((unsigned)(npairs / PPW)), block(64 * Q * PPW);
+    if ( io.src_rt == RealT::f32 )
+        launch_kernel<k_fwd_full_wave_pair<P, Q, float, PPW>>(grid, block, 0, st, static_cast<const float*>(io.src),
+                to_device(s.im), static_cast<float2*>(io.dst), to_device(s.om), s.nout, npairs, io.scale, tw);
+    else
+        launch_kernel<k_fwd_full_wave_pair<P, Q, __half, PPW>>(grid, block, 0, st, static_cast<const __half*>(io.src),
+                to_device(s.im), static_cast<float2*>(io.dst), to_device(s.om), s.nout, npairs, io.scale, tw);
+    return hipGetLastError( );
+}
+
 template <int P, int Q>
 static hipError_t go_full_wave_pair(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
-    const dim3 grid((unsigned)(s.nlines / 2)), block(64 * Q);
-    if ( io.src_rt == RealT::f32 )
-        launch_kernel<k_fwd_full_wave_pair<P, Q, float>>(grid, block, 0, st, static_cast<const float*>(io.src), to_device(s.im),
-                                                                  static_cast<float2*>(io.dst), to_device(s.om), s.nout,
-                                                                  s.nlines / 2, io.scale, tw);
-    else
-        launch_kernel<k_fwd_full_wave_pair<P, Q, __half>>(grid, block, 0, st, static_cast<const __half*>(io.src), to_device(s.im),
-                                                                   static_cast<float2*>(io.dst), to_device(s.om), s.nout,
-                                                                   s.nlines / 2, io.scale, tw);
-    return hipGetLastError( );
+    if constexpr ( Q <= 8 ) {
+        if ( env_int("FASTFFT_PAIR_GROUP", 2) == 2 && (s.nlines / 2) % 2 == 0 )
+            return go_full_wave_pair_g<P, Q, 2>(s, io, tw, st);
+    }
+    return go_full_wave_pair_g<P, Q, 1>(s, io, tw, st);
 }
 
 static bool full_wave_pair_covers(const Stage& s, const StageIO& io) {

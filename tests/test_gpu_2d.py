@@ -211,16 +211,21 @@ FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (25
              ((4096, 4096), (4096, 4096)), ((8192, 16), (8192, 16)), ((16, 8192), (32, 8192)), ((1000, 900), (1024, 1024)),
              ((256, 256), (64, 32)),
              # packed R2C rows: odd non-zero length on 512-point rows, a decrease from 1024 keeping 129 outputs
-             ((301, 50), (512, 64)), ((1024, 64), (256, 32))]
+             ((301, 50), (512, 64)), ((1024, 64), (256, 32)),
+             # full-length real rows as pairs: two pairs per workgroup (1024- and 2048-point rows), and an odd
+             # pair count (one pair per workgroup)
+             ((1024, 32), (1024, 32)), ((2048, 16), (2048, 16)), ((2048, 2), (2048, 2))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_packed", "no_pair", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_packed", "no_pair", "pair_group1", "generic"])
 @pytest.mark.parametrize("fin,fout", FWD_CASES)
 def test_fwd_random(fin, fout, mode, monkeypatch):
     """Forward transforms against the FP64 oracle with the default kernels (512- and 1024-point real
-    rows on the packed one-wave-per-row kernel, full-length real rows of 2048 points and up two per
-    workgroup as one complex line), without the packed rows, with one full-length row per workgroup,
-    and generic only."""
+    rows on the packed one-wave-per-row kernel, full-length real rows of 1024 points and up as pairs of
+    rows transformed as one complex line, two pairs per workgroup where the count allows), without the
+    packed rows, with one full-length row per workgroup, with one pair per workgroup, and generic only."""
+    if mode == "pair_group1":
+        monkeypatch.setenv("FASTFFT_PAIR_GROUP", "1")
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "no_packed":
