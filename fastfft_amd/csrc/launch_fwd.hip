@@ -210,8 +210,9 @@ static hipError_t go_full_wave(const Stage& s, const StageIO& io, const float2* 
 
 // Full-length real lines into a transposed spectrum (lines adjacent, even element
 // step, 16-byte aligned output): two lines per workgroup as one complex line.
-// PPW pairs per workgroup (FASTFFT_PAIR_GROUP: 1 or 2; 2 where the pair count allows
-// and a workgroup of 2 Q waves fits).
+// PPW pairs per workgroup (FASTFFT_PAIR_GROUP = 1, 2 or 4 caps it; by default 4 for
+// 4096-point rows, 2 for the other lengths where the pair count allows and a workgroup
+// of 2 Q waves fits).
 template <int P, int Q, int PPW>
 static hipError_t go_full_wave_pair_g(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     const int  npairs = s.nlines / 2;
@@ -228,7 +229,7 @@ static hipError_t go_full_wave_pair_g(const Stage& s, const StageIO& io, const f
 template <int P, int Q>
 static hipError_t go_full_wave_pair(const Stage& s, const StageIO& io, const float2* tw, hipStream_t st) {
     if constexpr ( Q <= 8 ) {
-        if ( env_int("FASTFFT_PAIR_GROUP", 2) == 2 && (s.nlines / 2) % 2 == 0 )
+        if ( env_int("FASTFFT_PAIR_GROUP", 4) >= 2 && (s.nlines / 2) % 2 == 0 )
             return go_full_wave_pair_g<P, Q, 2>(s, io, tw, st);
     }
     return go_full_wave_pair_g<P, Q, 1>(s, io, tw, st);
@@ -247,7 +248,11 @@ static hipError_t try_full_wave(const Stage& s, const StageIO& io, const float2*
         switch ( s.N ) {
             case 1024: return go_full_wave_pair<512, 2>(s, io, tw, st);
             case 2048: return go_full_wave_pair<512, 4>(s, io, tw, st);
-            case 4096: return go_full_wave_pair<512, 8>(s, io, tw, st);
+            case 4096:
+                // four pairs of 4096-point rows per workgroup on 1024-point sub-transforms (Q = 4)
+                if ( env_int("FASTFFT_PAIR_GROUP", 4) >= 4 && (s.nlines / 2) % 4 == 0 )
+                    return go_full_wave_pair_g<1024, 4, 4>(s, io, tw, st);
+                return go_full_wave_pair<512, 8>(s, io, tw, st);
             case 8192: return go_full_wave_pair<512, 16>(s, io, tw, st);
             default: break;
         }

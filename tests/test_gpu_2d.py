@@ -217,15 +217,16 @@ FWD_CASES = [((64, 64), (512, 512)), ((384, 384), (4096, 4096)), ((100, 60), (25
              ((1024, 32), (1024, 32)), ((2048, 16), (2048, 16)), ((2048, 2), (2048, 2))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_packed", "no_pair", "pair_group1", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_packed", "no_pair", "pair_group1", "pair_group2", "generic"])
 @pytest.mark.parametrize("fin,fout", FWD_CASES)
 def test_fwd_random(fin, fout, mode, monkeypatch):
     """Forward transforms against the FP64 oracle with the default kernels (512- and 1024-point real
     rows on the packed one-wave-per-row kernel, full-length real rows of 1024 points and up as pairs of
-    rows transformed as one complex line, two pairs per workgroup where the count allows), without the
-    packed rows, with one full-length row per workgroup, with one pair per workgroup, and generic only."""
-    if mode == "pair_group1":
-        monkeypatch.setenv("FASTFFT_PAIR_GROUP", "1")
+    rows transformed as one complex line, four pairs per workgroup for 4096-point rows and two for the
+    others where the count allows), without the packed rows, with one full-length row per workgroup,
+    with one and with at most two pairs per workgroup, and generic only."""
+    if mode.startswith("pair_group"):
+        monkeypatch.setenv("FASTFFT_PAIR_GROUP", mode[-1])
     if mode == "generic":
         monkeypatch.setenv("FASTFFT_NO_WAVE_ROWS", "1")
     if mode == "no_packed":
