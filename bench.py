@@ -82,8 +82,8 @@ def main() -> int:
                     help="also run one device copy of this many MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build runs no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The correlation shards by template; see the design notes.", file=sys.stderr)
+        print(f"bench.py measures one GPU only: the library has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "A batch would shard by template, one process per GPU; see the design notes.", file=sys.stderr)
         return 2
 
     import numpy as np
