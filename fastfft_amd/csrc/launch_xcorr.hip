@@ -49,9 +49,15 @@ static hipError_t go_xcorr_col(const Stage& s, const StageIO& io, const float2* 
         groups        = std::max(1, std::min(groups, io.nxitems));
     }
     XcorrItem one{io.src, io.dst, s.L, 0};
-    launch_kernel<k_xcorr_col<P, Q, RS, TImg>>(dim3((unsigned)blocks, (unsigned)groups), dim3(64 * Q), 0, st,
-            one, static_cast<const XcorrItem*>(io.xitems), io.nxitems, s.im.s1, s.im.es, static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M,
-            s.o, io.scale, tw, s.nlines - 1, pack);
+    const dim3 grid((unsigned)blocks, (unsigned)groups), block(64 * Q);
+    // one template per launch: the single-template form (FASTFFT_XCORR_ONE=0 keeps the loop form)
+    if ( ! io.xitems && env_int("FASTFFT_XCORR_ONE", 1) )
+        launch_kernel<k_xcorr_col<P, Q, RS, TImg, true>>(grid, block, 0, st, one, nullptr, 1, s.im.s1, s.im.es,
+                static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M, s.o, io.scale, tw, s.nlines - 1, pack);
+    else
+        launch_kernel<k_xcorr_col<P, Q, RS, TImg>>(grid, block, 0, st, one, static_cast<const XcorrItem*>(io.xitems),
+                io.nxitems, s.im.s1, s.im.es, static_cast<const TImg*>(io.img), s.gm.s1, s.om.s1, s.om.es, s.M, s.o, io.scale,
+                tw, s.nlines - 1, pack);
     return hipGetLastError( );
 }
 

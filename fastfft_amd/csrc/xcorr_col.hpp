@@ -76,7 +76,14 @@ struct ImgStage<__half2> {
 // per-lane input and output twiddles), and the wave-uniform twiddles
 // W_N^{64 i q} are scalar loads, so the only vector memory traffic is the
 // column data.
-template <int P, int Q, int RS, typename TImg>
+//
+// ONE (one template per launch, items == nullptr): the image values and the
+// per-register twiddles are formed where they are used instead of being held in
+// registers for a loop over templates, and the image column is read from memory
+// at its product (the Q waves read neighbouring words of the same lines) instead
+// of being staged in LDS first (the one-correlation y-pass of a 4096^2 image:
+// 35.4 -> 33.0 us).
+template <int P, int Q, int RS, typename TImg, bool ONE = false>
 __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <= 512 ? 4 : 1)))
         k_xcorr_col(XcorrItem one, const XcorrItem* __restrict__ items, int nitems, long lda, long a_es,
                     const TImg* __restrict__ img, long ldg, long ldo, long o_es, int M, int o, float intra_scale,
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
 
     // ---- stage: twiddle tables, image column
     stage_row_twiddles<P, Q>(s_twp, s_twn, twm);
-    if ( ! ends ) {
+    if ( ! ends && ! ONE ) {
         const TImg* gc = img + col * ldg;
 #pragma unroll
         for ( int r = 0; r < E / PER; r++ ) {
@@ -137,8 +144,8 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
     const int    omask = ((o % N) + N) % N;
     const float2 wo    = twm[N + ((omask * q) & (N - 1))];
     const float2 sigma = make_float2(wo.x * intra_scale, -wo.y * intra_scale);
-    float2       g[E];
-    if ( ! ends ) {
+    float2       g[ONE ? 1 : E];
+    if ( ! ends && ! ONE ) {
 #pragma unroll
         for ( int i = 0; i < E; i++ )
             g[i] = cmul(lds[dec_slot<P, Q>(Q * (t + T * i) + q)], sigma);
@@ -151,9 +158,16 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
     // neither copy holds the other's values across iterations.
     auto templates = [&](auto ends_c) {
         constexpr bool ENDS = decltype(ends_c)::value;
-        // W_N^{(t + 64 i) q}, held for all templates on the common path
-        float2 win[ENDS ? 1 : E];
-        if constexpr ( ! ENDS ) {
+        // W_N^{(t + 64 i) q}, held for all templates on the common path (formed at each
+        // use for one template)
+        float2 win[(ENDS || ONE) ? 1 : E];
+        auto   win_at = [&](int i) {
+            if constexpr ( ONE || ENDS )
+                return cmul(twm[N + ((T * i * q) & (N - 1))], lw);
+            else
+                return win[i];
+        };
+        if constexpr ( ! ENDS && ! ONE ) {
 #pragma unroll
             for ( int i = 0; i < E; i++ )
                 win[i] = cmul(twm[N + ((T * i * q) & (N - 1))], lw);
@@ -187,11 +201,17 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
                 if ( q != 0 ) {
 #pragma unroll
                     for ( int i = 0; i < E; i++ )
-                        v[i] = cmul(v[i], win[i]);
+                        v[i] = cmul(v[i], win_at(i));
                 }
             }
             fft.template run<-1>(v, wl, t);
-            if constexpr ( ! ENDS ) {
+            if constexpr ( ! ENDS && ONE ) {
+                const auto* gc = as_global(img) + col * ldg;
+#pragma unroll
+                for ( int i = 0; i < E; i++ )
+                    v[i] = cmulc(cmul(load_cpx(gc, (long)(Q * (t + T * i) + q)), sigma), v[i]);
+            }
+            else if constexpr ( ! ENDS ) {
 #pragma unroll
                 for ( int i = 0; i < E; i++ )
                     v[i] = cmulc(g[i], v[i]); // img * sigma * conj(X)
@@ -235,7 +255,7 @@ __global__ void __launch_bounds__(64 * Q) __attribute__((amdgpu_waves_per_eu(P <
                 if ( q != 0 ) {
 #pragma unroll
                     for ( int i = 0; i < E; i++ )
-                        v[(i + RS) % E] = cmulc(v[(i + RS) % E], win[i]);
+                        v[(i + RS) % E] = cmulc(v[(i + RS) % E], win_at(i));
                 }
             }
 #pragma unroll

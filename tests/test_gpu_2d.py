@@ -313,10 +313,13 @@ WAVE_CASES = [((48, 384), (64, 4096), (32, 512), (0, -256)), ((48, 384), (64, 40
               ((24, 1000), (32, 4096), (16, 1024), (0, -512)), ((16, 130), (32, 1024), (16, 256), (0, 128))]
 
 
-@pytest.mark.parametrize("mode", ["default", "no_pack", "generic"])
+@pytest.mark.parametrize("mode", ["default", "no_pack", "loop_form", "generic"])
 @pytest.mark.parametrize("tin,n,nout,origin", WAVE_CASES)
 def test_xcorr_column_kernel(tin, n, nout, origin, mode, monkeypatch):
-    env = {"default": {}, "no_pack": {"FASTFFT_XCORR_PACK": "0"}, "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
+    """The column kernel of one correlation in its single-template form (default), without the packed DC
+    and Nyquist columns, in the form that loops over templates (loop_form), and the generic kernel."""
+    env = {"default": {}, "no_pack": {"FASTFFT_XCORR_PACK": "0"}, "loop_form": {"FASTFFT_XCORR_ONE": "0"},
+           "generic": {"FASTFFT_NO_WAVE_XCORR": "1"}}[mode]
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     H.log(f"xcorr column {tin} in {n} -> {nout} at {origin} mode={mode}")
