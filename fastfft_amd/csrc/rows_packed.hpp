@@ -374,18 +374,37 @@ __global__ void __launch_bounds__(64 * kCmWaves)
         fft.template run<+1>(vr, tr, t); // vr[i] = z[t + 64 i]
         auto*              op   = as_global(static_cast<TOut*>(it.out)) + om.base(row);
         unsigned long long best = 0;
+        // every kept value inside the window, at unit stride on an aligned row: one store per pair
+        const bool whole = oN == 0 && M >= 2 * T * KEEP && om.es == 1 &&
+                           (reinterpret_cast<unsigned long>(op) & (2 * sizeof(TOut) - 1)) == 0;
+        if ( whole ) {
 #pragma unroll
-        for ( int i = 0; i < KEEP; i++ ) {
-            const int   n    = 2 * (t + T * i);
-            const float y[2] = {vr[i].x * post_scale, vr[i].y * post_scale};
+            for ( int i = 0; i < KEEP; i++ ) {
+                const int   n  = 2 * (t + T * i);
+                const float y0 = vr[i].x * post_scale, y1 = vr[i].y * post_scale;
+                store_real_pair(op, n, y0, y1);
+                if ( it.keys ) {
+                    const auto k0 = argmax_key(convert_real<TOut>(y0), (unsigned)(row * M + n));
+                    const auto k1 = argmax_key(convert_real<TOut>(y1), (unsigned)(row * M + n + 1));
+                    best          = k0 > best ? k0 : best;
+                    best          = k1 > best ? k1 : best;
+                }
+            }
+        }
+        else {
 #pragma unroll
-            for ( int h = 0; h < 2; h++ ) {
-                const int j = (n + h - oN + N) & (N - 1);
-                if ( j < M ) {
-                    store_real(op, (long)j * om.es, y[h]);
-                    if ( it.keys ) {
-                        const auto k = argmax_key(convert_real<TOut>(y[h]), (unsigned)(row * M + j));
-                        best         = k > best ? k : best;
+            for ( int i = 0; i < KEEP; i++ ) {
+                const int   n    = 2 * (t + T * i);
+                const float y[2] = {vr[i].x * post_scale, vr[i].y * post_scale};
+#pragma unroll
+                for ( int h = 0; h < 2; h++ ) {
+                    const int j = (n + h - oN + N) & (N - 1);
+                    if ( j < M ) {
+                        store_real(op, (long)j * om.es, y[h]);
+                        if ( it.keys ) {
+                            const auto k = argmax_key(convert_real<TOut>(y[h]), (unsigned)(row * M + j));
+                            best         = k > best ? k : best;
+                        }
                     }
                 }
             }
