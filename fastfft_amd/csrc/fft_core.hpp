@@ -156,6 +156,16 @@ __device__ __forceinline__ void store_real(global_t<__half>* p, long i, float v)
 // from an address-space-qualified object either)
 typedef float gvec2 __attribute__((ext_vector_type(2)));
 typedef float gvec4 __attribute__((ext_vector_type(4)));
+// values i and i + 1 of a real line as one load (i even, the line 2-element aligned)
+__device__ __forceinline__ float2 load_real_pair(const global_t<float>* p, long i) {
+    const gvec2 v = *reinterpret_cast<const global_t<gvec2>*>(p + i);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 load_real_pair(const global_t<__half>* p, long i) {
+    const unsigned u = *reinterpret_cast<const global_t<unsigned>*>(p + i);
+    return make_float2(__half2float(__ushort_as_half((unsigned short)(u & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(u >> 16))));
+}
 // values i and i + 1 of a real line as one store (i even, the line 2-element aligned)
 __device__ __forceinline__ void store_real_pair(global_t<float>* p, long i, float a, float b) {
     *reinterpret_cast<global_t<gvec2>*>(p + i) = gvec2{a, b};

@@ -230,6 +230,8 @@ __global__ void __launch_bounds__(64 * kCmWaves)
     FFT::template load_pass_table<NT>(ptv, twm, threadIdx.x);
     const float2 wt = load_cpx(as_global(twm), N + t); // W_N^t
     float2       v[RPW][E];
+    // even row pitch on an aligned buffer: each lane's two values as one load
+    const bool pairs = (it.in_ld & 1) == 0 && (reinterpret_cast<unsigned long>(it.in) & (2 * sizeof(TIn) - 1)) == 0;
 #pragma unroll
     for ( int r = 0; r < RPW; r++ ) {
         const int   row = row0 + w + kCmWaves * r;
@@ -237,7 +239,10 @@ __global__ void __launch_bounds__(64 * kCmWaves)
 #pragma unroll
         for ( int i = 0; i < E; i++ ) {
             const int n = 2 * (t + T * i);
-            v[r][i]     = make_float2(n < it.L ? load_real(rp, (long)n) : 0.f, n + 1 < it.L ? load_real(rp, (long)(n + 1)) : 0.f);
+            if ( pairs && n + 1 < it.L )
+                v[r][i] = load_real_pair(rp, (long)n);
+            else
+                v[r][i] = make_float2(n < it.L ? load_real(rp, (long)n) : 0.f, n + 1 < it.L ? load_real(rp, (long)(n + 1)) : 0.f);
         }
     }
     FFT::template store_pass_table<NT>(s_twp, ptv, threadIdx.x);
