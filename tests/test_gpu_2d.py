@@ -537,7 +537,7 @@ def test_log_names_kernel_launches():
     import subprocess
     import sys
 
-    H.log("FASTFFT_LOG=1 kernel launch lines of a 64 -> 512 -> 128 correlation")
+    H.log("FASTFFT_LOG=1 kernel launch lines of a 64 -> 512 -> 128 correlation and a batch of two")
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     code = ("import torch\n"
             "import fastfft_amd as F\n"
@@ -550,6 +550,11 @@ def test_log_names_kernel_launches():
             "img = torch.ones(ft.ReturnFwdOutputMemorySize(), device=d)\n"
             "out = torch.zeros(ft.ReturnInvOutputMemorySize(), device=d)\n"
             "ft.FwdImageInvFFT(a, img, out)\n"
+            "torch.cuda.synchronize()\n"
+            "ts = [a.clone(), a.clone()]\n"
+            "outs = [torch.zeros_like(out), torch.zeros_like(out)]\n"
+            "ft.SetBatch(ts, [(64, 64), (64, 64)], outs)\n"
+            "ft.FwdImageInvFFTBatch(img)\n"
             "torch.cuda.synchronize()\n")
     env = dict(os.environ, FASTFFT_LOG="1")
     r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=180)
@@ -565,3 +570,6 @@ def test_log_names_kernel_launches():
         names.append(m.group(1))
     # the twiddle table once, then one kernel per stage of the three-stage correlation
     assert names[0] == "k_twiddle_init" and len(names) >= 4
+    # the batch form, logged when set, and its launches (those of the tuning runs and the capture)
+    assert "fastfft: plan batch templates=2" in r.stderr
+    assert len(names) > 4
