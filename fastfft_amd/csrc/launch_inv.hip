@@ -91,11 +91,12 @@ hipError_t launch_c2r_stats_batch(const Stage& s, const void* items, int n, floa
                                   const BatchStats& bs, hipStream_t st) {
     if ( ! c2r_stats_covers(s) || n < 1 )
         return hipErrorNotSupported;
-    // template groups spread a chunk over the GPU when the scratch is there: 4 groups of at
-    // least 4 templates (fewer, larger groups write fewer partial maps), or groups of
-    // FASTFFT_STATS_PER_GROUP templates, at most kStatsGroups
+    // template groups spread a chunk over the GPU when the scratch is there: up to 4 groups of
+    // at least 4 templates, and groups of 8 for larger chunks (64 templates: 8 groups of 8, 1 %
+    // above 4 of 16 although twice the partial maps), or groups of FASTFFT_STATS_PER_GROUP
+    // templates; at most kStatsGroups
     const int tpg = env_int("FASTFFT_STATS_PER_GROUP", 0);
-    int       G   = tpg > 0 ? (n + tpg - 1) / tpg : std::min(4, (n + 3) / 4);
+    int       G   = tpg > 0 ? (n + tpg - 1) / tpg : std::max(std::min(4, (n + 3) / 4), (n + 7) / 8);
     G             = bs.part ? std::max(1, std::min(kStatsGroups, G)) : 1;
     const int per = (n + G - 1) / G;
     G             = (n + per - 1) / per; // no empty group
