@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(64 * Q)
     constexpr int T = 64, E = P / T, N = P * Q, NT = T * Q, H = N / 2;
     using FFT       = WaveFFT<P>;
     static_assert(Q * FFT::SLOTS >= N, "a line must fit the exchange buffers");
-    __shared__ float2 lds[Q * FFT::SLOTS];
+    __shared__ __align__(16) float2 lds[Q * FFT::SLOTS];
     __shared__ float2 s_twp[P];
     const int     q    = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     const int     t    = threadIdx.x % T;
@@ -110,6 +110,32 @@ __global__ void __launch_bounds__(64 * Q)
     const int oN = ((o % N) + N) % N;
     const int oP = oN & (P - 1);
     TOut*     op = out + om.base(line);
+    // complex unit-stride outputs from an even origin on 16-byte boundaries: two outputs per
+    // thread, their twiddles, shares and stores as 16-byte accesses
+    if constexpr ( ! REAL_OUT ) {
+        if ( om.es == 1 && (oN & 1) == 0 && (M & 1) == 0 && (om.base(line) & 1) == 0 &&
+             (reinterpret_cast<unsigned long>(out) & 15) == 0 ) {
+            const auto* lds4 = reinterpret_cast<const float4*>(lds);
+            for ( int r = 0; 2 * r * NT < M; r++ ) {
+                const int j = 2 * ((int)threadIdx.x + NT * r);
+                if ( j >= M )
+                    break;
+                const float4 w2 = load_cpx_pair(as_global(twm) + N, ((oN + j) & (N - 1)) >> 1); // W_N^{o + j}, W_N^{o + j + 1}
+                const int    m  = (oP + j) & (P - 1);
+                float4       a  = lds4[((Q - 1) * FFT::SLOTS + m) >> 1];
+#pragma unroll
+                for ( int qq = Q - 2; qq >= 0; qq-- ) {
+                    const float4 u  = lds4[(qq * FFT::SLOTS + m) >> 1];
+                    const float2 a0 = cmulc(make_float2(a.x, a.y), make_float2(w2.x, w2.y));
+                    const float2 a1 = cmulc(make_float2(a.z, a.w), make_float2(w2.z, w2.w));
+                    a               = make_float4(a0.x + u.x, a0.y + u.y, a1.x + u.z, a1.y + u.w);
+                }
+                store_cpx_pair(as_global(op), j >> 1, make_float2(a.x * post_scale, a.y * post_scale),
+                               make_float2(a.z * post_scale, a.w * post_scale));
+            }
+            return;
+        }
+    }
     for ( int r = 0; r * NT < M; r++ ) {
         const int j = (int)threadIdx.x + NT * r;
         if ( j >= M )
